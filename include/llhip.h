@@ -1,0 +1,93 @@
+/*
+ * llhip.h -- C ABI of the MI355X Lucas-Lehmer engine (libllhip.so).
+ *
+ * Plain scalars, caller-owned host arrays, int status codes; no exception crosses the
+ * boundary.  All device memory is owned by the library.  Calls are thread-safe: GPU work
+ * is serialised on one HIP stream per device (the reference's CUDA module serialises on the
+ * default stream, cuda/ll_ntt.cu:229-231).  The pybind11 module `lucaslehmer` releases the
+ * GIL around every call, as the reference does (lucaslehmer.cpp:38, :280).
+ *
+ * Transforms: LLH_FP64 = FP64 Crandall-Fagin IBDWT (real FFT, per-iteration round-off
+ * check, rollback to 2N above 0.4); LLH_NTT = exact Goldilocks-field IBDWT.
+ * Res64 is always the low 64 bits of the canonical residue in [0, 2^p - 1).
+ */
+#ifndef LLHIP_H
+#define LLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LLH_OK 0
+#define LLH_ERR_ARG -1      /* invalid argument (bad transform, NULL output, ...) */
+#define LLH_ERR_HIP -2      /* HIP runtime error; llh_last_error() has the text */
+#define LLH_ERR_UNSUPPORTED -3
+
+#define LLH_FP64 0
+#define LLH_NTT 1
+
+/* per-exponent status values written by llh_batch */
+#define LLH_ST_OK 0         /* ran min(p-2, max_iters) squarings; verdict and res64 valid */
+#define LLH_ST_ROUNDOFF 1   /* FP64 round-off stayed above 0.4 up to the longest length */
+#define LLH_ST_NOT_RUN 2    /* no supported transform length (cf. seconds < 0 in cuda/ll_ntt.cu:256-263) */
+#define LLH_ST_TRIVIAL 3    /* p < 3 or even: answered by lucaslehmer.cpp:30-32's rules */
+
+/* Select the device (default 0) and create the library's stream.  No reference counterpart:
+ * the reference's CUDA module uses the current device implicitly. */
+int llh_init(int device);
+
+/* Text of the last error on this thread ("" if none). */
+const char *llh_last_error(void);
+
+/* Device name, compute units, global memory bytes, gfx arch name.
+ * Replaces device_info() cuda/ll_ntt.cu:269-272. */
+int llh_device_info(char *name, size_t name_len, int *compute_units, uint64_t *global_mem_bytes,
+                    char *arch, size_t arch_len);
+
+/* Conservative FP64 length rule: smallest {1,3,5}*16*2^k with p/N <= max_bpw
+ * (max_bpw <= 0: the current bits-per-word setting).  Replaces fft_length lucaslehmer.cpp:244-254. */
+uint64_t llh_fft_length(uint64_t p, double max_bpw);
+
+/* Length the batched engine would use for p with `transform` (0 = not in the batched engine).
+ * For LLH_NTT this extends ntt_length cuda/ll_ntt.cu:176-180 down to N = 16. */
+uint64_t llh_batch_length(uint64_t p, int transform);
+
+/* Reference NTT length rule (N in 64..8192, 0 = too large).  cuda/ll_ntt.cu:176-180. */
+uint64_t llh_ntt_length(uint64_t p);
+
+/* Largest exponent `transform` handles in the batched engine.  cuda/ll_ntt.cu:279. */
+uint64_t llh_max_exponent(int transform);
+
+/* Bits-per-word cap of the FP64 length rule (default 18).
+ * Replaces set_fft_bits_per_word lucaslehmer.cpp:368-369 (atomic here, not a plain static). */
+int llh_set_fft_bits_per_word(double bits);
+double llh_get_fft_bits_per_word(void);
+
+/* Many exponents in one persistent launch.  Replaces lucas_lehmer_batch cuda/ll_ntt.cu:259-267
+ * (and the per-thread engine calls of PerfNumMultiCLL.py:189-193).  max_iters = 0 runs the full
+ * p-2 squarings.  Outputs are indexed like ps[] (the reference returns them grouped by length);
+ * any output pointer except verdict/res64/status may be NULL.  seconds = device time of that
+ * exponent; n_used = transform length (digits) actually used (after rollbacks). */
+int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, int32_t *verdict,
+              uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status);
+
+/* Device time of the kernel launches made by the last llh_batch / llh_lucas_lehmer call
+ * (HIP events on the library's stream) and how many launches it took (1 + rollback launches).
+ * No reference counterpart: measurement hook for the roofline report. */
+int llh_last_launch_stats(double *kernel_ms, uint32_t *launches);
+
+/* One exponent.  transform = LLH_NTT replaces lucas_lehmer_n lucaslehmer.cpp:29-56 (the exact
+ * engine; Res64 of gmp_res64_after :364-365); LLH_FP64 replaces lucas_lehmer_fft_raw
+ * lucaslehmer.cpp:256-329 (maxerr of fft_info :337-342 / fft_res64_after :361-363).
+ * status as in llh_batch. */
+int llh_lucas_lehmer(uint64_t p, uint64_t max_iters, int transform, int32_t *verdict,
+                     uint64_t *res64, double *maxerr, uint32_t *n_used, int32_t *status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LLHIP_H */

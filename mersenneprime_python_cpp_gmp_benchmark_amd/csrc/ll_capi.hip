@@ -1,0 +1,448 @@
+// ll_capi.hip -- the C ABI (include/llhip.h): device state, table cache, batch orchestration.
+//
+// Host-side counterpart of cuda/ll_ntt.cu:198-267 (run_batch / lucas_lehmer_batch): group
+// exponents by transform length, build the length's tables once (kept for the process, as
+// the reference keeps FFTW plans, lucaslehmer.cpp:228-242), one persistent launch for the
+// whole list, results scattered back in input order.  FP64 exponents whose round-off passes
+// 0.4 are re-queued at twice the length (rollback) in a follow-up launch.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/llhip.h"
+#include "ll_internal.hpp"
+
+namespace llh {
+
+hipError_t launch_batch(int transform, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
+                        BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream);
+int batch_blocks_per_cu(int transform);
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+
+struct TableSet {
+    void *tw = nullptr, *twi = nullptr, *wf = nullptr, *wi = nullptr;
+};
+
+struct DeviceState {
+    int id = -1;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    double wclk_hz = 1e8;
+    int bpc[2] = {0, 0};
+    std::map<std::pair<int, int>, TableSet> tables;
+    void *d_items = nullptr, *d_out = nullptr, *d_cls = nullptr, *d_cnt = nullptr;
+    size_t cap_items = 0, cap_cls = 0;
+};
+
+static std::recursive_mutex g_mx;
+static DeviceState g_dev;
+static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
+static u32 g_last_launches = 0;
+static std::atomic<double> g_bpw{18.0};
+
+// batched engine limits
+static const int kFp64MinLog = 5, kFp64MaxLog = 13;   // N = 32 .. 8192 (M = 16 .. 4096)
+static const int kNttMinLog = 4, kNttMaxLog = 12;     // N = 16 .. 4096
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            set_error(std::string("HIP: ") + hipGetErrorString(e_) + " at " #x);         \
+            return LLH_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+static int init_locked(int device) {
+    if (g_dev.id == device) return LLH_OK;
+    if (g_dev.id >= 0) {
+        set_error("llh_init: already initialised on another device");
+        return LLH_ERR_ARG;
+    }
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("llh_init: no such device");
+        return LLH_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    hipDeviceProp_t pr;
+    HIPCHK(hipGetDeviceProperties(&pr, device));
+    g_dev.cus = pr.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        g_dev.wclk_hz = (double)khz * 1e3;
+    g_dev.bpc[0] = batch_blocks_per_cu(kFP64);
+    g_dev.bpc[1] = batch_blocks_per_cu(kNTT);
+    g_dev.id = device;
+    return LLH_OK;
+}
+
+static int ensure_init() {
+    if (g_dev.id >= 0) {
+        HIPCHK(hipSetDevice(g_dev.id));
+        return LLH_OK;
+    }
+    return init_locked(0);
+}
+
+// cos(2 pi j / n), sin(2 pi j / n) with an exact quadrant/octant reduction, long double.
+static void sincos_frac(u64 j, u64 n, long double &c, long double &s) {
+    j %= n;
+    const long double HALFPI = 1.57079632679489661923132169163975144L;
+    u64 q = (4 * j) / n;               // quadrant
+    u64 rem = 4 * j - q * n;           // angle = (q + rem/n) * pi/2
+    long double cs, sn;
+    if (2 * rem <= n) {
+        long double phi = HALFPI * (long double)rem / (long double)n;
+        cs = cosl(phi);
+        sn = sinl(phi);
+    } else {
+        long double phi = HALFPI * (long double)(n - rem) / (long double)n;
+        cs = sinl(phi);
+        sn = cosl(phi);
+    }
+    switch (q) {
+        case 0: c = cs; s = sn; break;
+        case 1: c = -sn; s = cs; break;
+        case 2: c = -cs; s = -sn; break;
+        default: c = sn; s = -cs; break;
+    }
+}
+
+// Tables of one (transform, log_n).  Caller holds g_mx.
+static int get_tables(int transform, int log_n, TableSet &out) {
+    auto key = std::make_pair(transform, log_n);
+    auto it = g_dev.tables.find(key);
+    if (it != g_dev.tables.end()) {
+        out = it->second;
+        return LLH_OK;
+    }
+    const u64 N = (u64)1 << log_n;
+    TableSet ts;
+    if (transform == kFP64) {
+        const u64 M = N / 2;
+        std::vector<double> tw(2 * M), wf(N), wi(N);
+        for (u64 j = 0; j < M; ++j) {
+            long double c, s;
+            sincos_frac(j, M, c, s);
+            tw[2 * j] = (double)c;
+            tw[2 * j + 1] = (double)(-s);          // w_M^j = exp(-2 pi i j / M)
+        }
+        for (u64 e = 0; e < N; ++e) {
+            long double w = exp2l((long double)e / (long double)N);
+            wf[e] = (double)w;
+            wi[e] = (double)(1.0L / (w * (long double)M));
+        }
+        HIPCHK(hipMalloc(&ts.tw, tw.size() * 8));
+        HIPCHK(hipMalloc(&ts.wf, N * 8));
+        HIPCHK(hipMalloc(&ts.wi, N * 8));
+        HIPCHK(hipMemcpy(ts.tw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.wf, wf.data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.wi, wi.data(), N * 8, hipMemcpyHostToDevice));
+    } else {
+        const u64 w = gl_root_unity_aligned(N), wv = gl_inv_h(w);
+        const u64 r2 = gl_root2_h(N), r2i = gl_inv_h(r2), ninv = gl_inv_h(N);
+        if (!w || !r2) {
+            set_error("no Goldilocks roots for this length");
+            return LLH_ERR_UNSUPPORTED;
+        }
+        std::vector<u64> tw(N), twi(N), wf(N), wi(N);
+        u64 a = 1, b = 1, c = 1, d = ninv;
+        for (u64 j = 0; j < N; ++j) {
+            tw[j] = a; twi[j] = b; wf[j] = c; wi[j] = d;
+            a = gl_mul_h(a, w); b = gl_mul_h(b, wv); c = gl_mul_h(c, r2); d = gl_mul_h(d, r2i);
+        }
+        HIPCHK(hipMalloc(&ts.tw, N * 8));
+        HIPCHK(hipMalloc(&ts.twi, N * 8));
+        HIPCHK(hipMalloc(&ts.wf, N * 8));
+        HIPCHK(hipMalloc(&ts.wi, N * 8));
+        HIPCHK(hipMemcpy(ts.tw, tw.data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.twi, twi.data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.wf, wf.data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.wi, wi.data(), N * 8, hipMemcpyHostToDevice));
+    }
+    g_dev.tables[key] = ts;
+    out = ts;
+    return LLH_OK;
+}
+
+// log2 of the batched engine's length for p (0 = not in the batched engine).
+static int batch_log_n(u64 p, int transform) {
+    if (p < 64) return 0;
+    // the kernels index digits with 32-bit products j * p, j < N
+    auto fits = [p](int l) { return (((u64)1 << l) + 2) * p < ((u64)1 << 32); };
+    if (transform == kFP64) {
+        const double bpw = g_bpw.load();
+        for (int l = kFp64MinLog; l <= kFp64MaxLog; ++l)
+            if ((double)p / (double)((u64)1 << l) <= bpw) return fits(l) ? l : 0;
+        return 0;
+    }
+    for (int l = kNttMinLog; l <= kNttMaxLog; ++l) {
+        const u64 N = (u64)1 << l;
+        if ((p + N - 1) / N <= (u64)ntt_max_bits(N)) return fits(l) ? l : 0;
+    }
+    return 0;
+}
+
+static int groups_per_wg(int log_m) {
+    // G = M / R with R = 16 for M >= 256, R = 8 for M = 64, 128, R = 4 below
+    const int M = 1 << log_m;
+    const int R = M >= 256 ? 16 : (M >= 64 ? 8 : 4);
+    return 256 / (M / R);
+}
+
+static int grow(void **ptr, size_t &cap, size_t need) {
+    if (need <= cap) return LLH_OK;
+    if (*ptr) HIPCHK(hipFree(*ptr));
+    *ptr = nullptr;
+    size_t c = std::max(need, cap * 2);
+    HIPCHK(hipMalloc(ptr, c));
+    cap = c;
+    return LLH_OK;
+}
+
+struct Pending {
+    u64 p;
+    u64 iters;
+    size_t idx;
+    int log_n;
+};
+
+// One persistent launch over `work`; fills res[i] for work[i].
+static int run_launch(int transform, std::vector<Pending> &work, std::vector<BatchResult> &res) {
+    const size_t n = work.size();
+    res.assign(n, BatchResult());
+    if (!n) return LLH_OK;
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    // longest first: by length descending, p descending; the one-word lane (log_n 0) last
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        if (work[a].log_n != work[b].log_n) return work[a].log_n > work[b].log_n;
+        return work[a].p > work[b].p;
+    });
+    std::vector<BatchItem> items(n);
+    std::vector<ClassDesc> cls;
+    size_t waves = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const Pending &w = work[order[k]];
+        items[k].p = (u32)w.p;
+        items[k].log_n = (u32)w.log_n;
+        items[k].iters = w.iters;
+        items[k].slot = (u32)order[k];
+        items[k].pad = 0;
+        if (cls.empty() || cls.back().log_n != w.log_n) {
+            ClassDesc cd;
+            std::memset(&cd, 0, sizeof(cd));
+            cd.log_n = w.log_n;
+            cd.log_m = w.log_n == 0 ? 0 : (transform == kFP64 ? w.log_n - 1 : w.log_n);
+            cd.first = (u32)k;
+            cd.count = 0;
+            if (w.log_n) {
+                TableSet ts;
+                int rc = get_tables(transform, w.log_n, ts);
+                if (rc) return rc;
+                cd.tw = ts.tw; cd.twi = ts.twi; cd.wf = ts.wf; cd.wi = ts.wi;
+            }
+            cls.push_back(cd);
+        }
+        cls.back().count++;
+    }
+    for (auto &c : cls) waves += c.log_m ? (c.count + groups_per_wg(c.log_m) - 1) / groups_per_wg(c.log_m) : 1;
+    int rc;
+    if ((rc = grow(&g_dev.d_items, g_dev.cap_items, n * sizeof(BatchItem) + n * sizeof(BatchResult) + 64)))
+        return rc;
+    if ((rc = grow(&g_dev.d_cls, g_dev.cap_cls, cls.size() * (sizeof(ClassDesc) + 16) + 64))) return rc;
+    BatchItem *d_items = (BatchItem *)g_dev.d_items;
+    BatchResult *d_out = (BatchResult *)((char *)g_dev.d_items + ((n * sizeof(BatchItem) + 63) / 64) * 64);
+    ClassDesc *d_cls = (ClassDesc *)g_dev.d_cls;
+    u32 *d_cnt = (u32 *)((char *)g_dev.d_cls + ((cls.size() * sizeof(ClassDesc) + 63) / 64) * 64);
+    hipStream_t st = g_dev.stream;
+    HIPCHK(hipMemcpyAsync(d_items, items.data(), n * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(ClassDesc), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(d_cnt, 0, cls.size() * sizeof(u32), st));
+    const int tr = transform == kNTT ? 1 : 0;
+    int grid = (int)std::min<size_t>(std::max<size_t>(waves, 1), (size_t)g_dev.cus * g_dev.bpc[tr]);
+    hipEvent_t ev0, ev1;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventRecord(ev0, st));
+    HIPCHK(launch_batch(transform, d_cls, (int)cls.size(), d_items, d_out, d_cnt, grid, st));
+    HIPCHK(hipEventRecord(ev1, st));
+    HIPCHK(hipMemcpyAsync(res.data(), d_out, n * sizeof(BatchResult), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    g_last_kernel_ms += ms;
+    g_last_launches += 1;
+    return LLH_OK;
+}
+
+static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int32_t *verdict,
+                      uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status) {
+    if (transform != kFP64 && transform != kNTT) {
+        set_error("transform must be LLH_FP64 or LLH_NTT");
+        return LLH_ERR_ARG;
+    }
+    if (n && (!ps || !verdict || !res64 || !status)) {
+        set_error("NULL argument");
+        return LLH_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    int rc = ensure_init();
+    if (rc) return rc;
+    g_last_kernel_ms = 0.0;
+    g_last_launches = 0;
+    std::vector<Pending> work;
+    std::vector<double> secs(n, 0.0), errs(n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        const u64 p = ps[i];
+        verdict[i] = 0;
+        res64[i] = 0;
+        status[i] = kStTrivial;
+        if (n_used) n_used[i] = 0;
+        if (p < 2) continue;                                   // lucaslehmer.cpp:30
+        if (p == 2) { verdict[i] = 1; continue; }              // :31 (M_2 = 3 is prime)
+        if (p % 2 == 0) continue;                              // :32
+        if (p >= ((u64)1 << 31)) { status[i] = kStNotRun; continue; }
+        const u64 it = max_iters ? std::min<u64>(max_iters, p - 2) : p - 2;
+        const int ln = batch_log_n(p, transform);
+        if (p >= 64 && ln == 0) { status[i] = kStNotRun; continue; }
+        work.push_back(Pending{p, it, i, ln});
+    }
+    const int max_log = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
+    while (!work.empty()) {
+        std::vector<BatchResult> res;
+        rc = run_launch(transform, work, res);
+        if (rc) return rc;
+        std::vector<Pending> again;
+        for (size_t k = 0; k < work.size(); ++k) {
+            const Pending &w = work[k];
+            const BatchResult &r = res[k];
+            secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
+            errs[w.idx] = std::max(errs[w.idx], r.maxerr);
+            if (r.status == kStRoundoff && w.log_n < max_log &&
+                (((u64)2 << w.log_n) + 2) * w.p < ((u64)1 << 32)) {
+                again.push_back(Pending{w.p, w.iters, w.idx, w.log_n + 1});   // rollback at 2N
+                continue;
+            }
+            verdict[w.idx] = r.verdict;
+            res64[w.idx] = r.res64;
+            status[w.idx] = r.status;
+            if (n_used) n_used[w.idx] = w.log_n ? (1u << w.log_n) : 1u;
+        }
+        work.swap(again);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (seconds) seconds[i] = secs[i];
+        if (maxerr) maxerr[i] = errs[i];
+    }
+    return LLH_OK;
+}
+
+}  // namespace llh
+
+using namespace llh;
+
+extern "C" {
+
+int llh_init(int device) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    return init_locked(device);
+}
+
+const char *llh_last_error(void) { return g_err.c_str(); }
+
+int llh_device_info(char *name, size_t name_len, int *compute_units, uint64_t *global_mem_bytes, char *arch,
+                    size_t arch_len) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipDeviceProp_t pr;
+    HIPCHK(hipGetDeviceProperties(&pr, g_dev.id));
+    if (name && name_len) {
+        std::strncpy(name, pr.name, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (arch && arch_len) {
+        std::strncpy(arch, pr.gcnArchName, arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    if (compute_units) *compute_units = pr.multiProcessorCount;
+    if (global_mem_bytes) *global_mem_bytes = (uint64_t)pr.totalGlobalMem;
+    return LLH_OK;
+}
+
+uint64_t llh_fft_length(uint64_t p, double max_bpw) {
+    return fft_length_conservative(p, max_bpw > 0 ? max_bpw : g_bpw.load());
+}
+
+uint64_t llh_batch_length(uint64_t p, int transform) {
+    if (p < 64) return p >= 3 ? 1 : 0;
+    int l = batch_log_n(p, transform);
+    return l ? ((uint64_t)1 << l) : 0;
+}
+
+uint64_t llh_ntt_length(uint64_t p) { return ntt_length_ref(p); }
+
+uint64_t llh_max_exponent(int transform) {
+    if (transform == kNTT) return ((uint64_t)1 << kNttMaxLog) * (uint64_t)ntt_max_bits((uint64_t)1 << kNttMaxLog);
+    return (uint64_t)std::floor(g_bpw.load() * (double)((uint64_t)1 << kFp64MaxLog));
+}
+
+int llh_set_fft_bits_per_word(double bits) {
+    if (!(bits > 0) || bits > 64) {
+        set_error("bits per word must be in (0, 64]");
+        return LLH_ERR_ARG;
+    }
+    g_bpw.store(bits);
+    return LLH_OK;
+}
+
+double llh_get_fft_bits_per_word(void) { return g_bpw.load(); }
+
+int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, int32_t *verdict, uint64_t *res64,
+              double *seconds, double *maxerr, uint32_t *n_used, int32_t *status) {
+    try {
+        return batch_impl(ps, n, max_iters, transform, verdict, res64, seconds, maxerr, n_used, status);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return LLH_ERR_HIP;
+    }
+}
+
+int llh_last_launch_stats(double *kernel_ms, uint32_t *launches) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    if (kernel_ms) *kernel_ms = g_last_kernel_ms;
+    if (launches) *launches = g_last_launches;
+    return LLH_OK;
+}
+
+int llh_lucas_lehmer(uint64_t p, uint64_t max_iters, int transform, int32_t *verdict, uint64_t *res64,
+                     double *maxerr, uint32_t *n_used, int32_t *status) {
+    double sec = 0, err = 0;
+    uint32_t nu = 0;
+    int32_t v = 0, st = 0;
+    uint64_t r = 0;
+    int rc = llh_batch(&p, 1, max_iters, transform, &v, &r, &sec, &err, &nu, &st);
+    if (rc) return rc;
+    if (verdict) *verdict = v;
+    if (res64) *res64 = r;
+    if (maxerr) *maxerr = err;
+    if (n_used) *n_used = nu;
+    if (status) *status = st;
+    return LLH_OK;
+}
+
+}  // extern "C"

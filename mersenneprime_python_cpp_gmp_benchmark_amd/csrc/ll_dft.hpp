@@ -1,0 +1,149 @@
+// ll_dft.hpp -- radix-2/4/8/16 butterflies and one register-blocked Stockham pass.
+//
+// A length-M transform is run by G threads that each hold R = M/G elements in registers.
+// Pass i has radix r_i (r_i divides R); a thread runs R/r_i butterflies per pass.  The
+// Stockham ("autosort") formulation keeps the output in natural order:
+//
+//   pass (radix r, L = product of earlier radices), butterfly index i in [0, M/r):
+//     k = i mod L;  x_s = A[i + s M/r] * w_{rL}^{s k};  y = DFT_r(x);
+//     B[(i - k) r + k + q L] = y_q
+//
+// With r_first = r_last = R, thread t's inputs to the first pass are elements
+// t + G s (s < R) and its outputs of the last pass are elements t + G q: both are the
+// same "column" layout, so the digits stay in registers between iterations and only the
+// middle hand-offs go through LDS.  LDS indices are padded by one element per sixteen so
+// the stride-16 stores of a radix-16 pass do not pile onto one bank.
+#pragma once
+#include "ll_common.hpp"
+
+LL_HD constexpr int lds_pad(int i) { return i + (i >> 4); }
+
+template <class F, bool INV>
+LL_DEV void dft2(typename F::T &a0, typename F::T &a1) {
+    typename F::T t = F::sub(a0, a1);
+    a0 = F::add(a0, a1);
+    a1 = t;
+}
+
+template <class F, bool INV>
+LL_DEV void dft4(typename F::T &a0, typename F::T &a1, typename F::T &a2, typename F::T &a3) {
+    typedef typename F::T T;
+    T t0 = F::add(a0, a2), t1 = F::sub(a0, a2);
+    T t2 = F::add(a1, a3);
+    T t3 = F::template mul_w16<4, INV>(F::sub(a1, a3));
+    a0 = F::add(t0, t2);
+    a2 = F::sub(t0, t2);
+    a1 = F::add(t1, t3);
+    a3 = F::sub(t1, t3);
+}
+
+// y[q] = sum_s x[s] w_r^{s q} in place, natural order in and out.
+template <class F, int r, bool INV>
+LL_DEV void dft_r(typename F::T *x) {
+    typedef typename F::T T;
+    if constexpr (r == 1) {
+        return;
+    } else if constexpr (r == 2) {
+        dft2<F, INV>(x[0], x[1]);
+    } else if constexpr (r == 4) {
+        dft4<F, INV>(x[0], x[1], x[2], x[3]);
+    } else if constexpr (r == 8) {
+        // s = b + 2a, q = c + 4d:  Z_b = DFT4(x[b + 2a]),  Z_b[c] *= w8^{bc},  y = DFT2 over b
+        T z0[4] = {x[0], x[2], x[4], x[6]};
+        T z1[4] = {x[1], x[3], x[5], x[7]};
+        dft4<F, INV>(z0[0], z0[1], z0[2], z0[3]);
+        dft4<F, INV>(z1[0], z1[1], z1[2], z1[3]);
+        z1[1] = F::template mul_w16<2, INV>(z1[1]);
+        z1[2] = F::template mul_w16<4, INV>(z1[2]);
+        z1[3] = F::template mul_w16<6, INV>(z1[3]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            x[c] = F::add(z0[c], z1[c]);
+            x[c + 4] = F::sub(z0[c], z1[c]);
+        }
+    } else if constexpr (r == 16) {
+        // s = b + 4a, q = c + 4d:  Z_b = DFT4(x[b + 4a]),  Z_b[c] *= w16^{bc},  y = DFT4 over b
+        T z[4][4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            z[b][0] = x[b]; z[b][1] = x[b + 4]; z[b][2] = x[b + 8]; z[b][3] = x[b + 12];
+            dft4<F, INV>(z[b][0], z[b][1], z[b][2], z[b][3]);
+        }
+        z[1][1] = F::template mul_w16<1, INV>(z[1][1]);
+        z[1][2] = F::template mul_w16<2, INV>(z[1][2]);
+        z[1][3] = F::template mul_w16<3, INV>(z[1][3]);
+        z[2][1] = F::template mul_w16<2, INV>(z[2][1]);
+        z[2][2] = F::template mul_w16<4, INV>(z[2][2]);
+        z[2][3] = F::template mul_w16<6, INV>(z[2][3]);
+        z[3][1] = F::template mul_w16<3, INV>(z[3][1]);
+        z[3][2] = F::template mul_w16<6, INV>(z[3][2]);
+        z[3][3] = F::template mul_w16<9, INV>(z[3][3]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            dft4<F, INV>(z[0][c], z[1][c], z[2][c], z[3][c]);
+            x[c] = z[0][c]; x[c + 4] = z[1][c]; x[c + 8] = z[2][c]; x[c + 12] = z[3][c];
+        }
+    } else {
+        static_assert(r == 16, "unsupported radix");
+    }
+}
+
+// One Stockham pass over the LDS buffer `lds` (padded, M elements) for a group of G
+// threads; t is the thread's index inside its group.  `tw` holds w_M^j, j < M (for the
+// inverse transform the C64 path conjugates, the GL path is handed the inverse table).
+// FIRST: inputs come from registers;  LAST: outputs stay in registers.
+template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV>
+LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t,
+                          const typename F::W *__restrict__ tw) {
+    static_assert(R % r == 0, "radix must divide the per-thread count");
+    static_assert(!FIRST || r == R, "first pass must use the full per-thread radix");
+    static_assert(!LAST || (r == R && L * r == M), "last pass must use the full radix");
+    constexpr int NB = R / r;
+    constexpr int STRIDE = M / r;
+    if constexpr (!FIRST) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int s = 0; s < r; ++s) x[b * r + s] = lds[lds_pad(t + G * b + s * STRIDE)];
+    }
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int i = t + G * b;
+            const int k = i % L;
+            constexpr int TSTEP = M / (r * L);
+#pragma unroll
+            for (int s = 1; s < r; ++s)
+                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], tw[s * k * TSTEP]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) dft_r<F, r, INV>(&x[b * r]);
+    if constexpr (!LAST) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int i = t + G * b;
+            const int k = i % L;
+#pragma unroll
+            for (int q = 0; q < r; ++q) lds[lds_pad((i - k) * r + k + q * L)] = x[b * r + q];
+        }
+    }
+}
+
+// Whole transform: two or three passes (R3 == 0 means two).
+template <class F, int M, int G, int R, int R1, int R2, int R3, bool INV>
+LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t,
+                      const typename F::W *__restrict__ tw) {
+    if constexpr (R3 == 0) {
+        static_assert(R1 * R2 == M, "radices must multiply to M");
+        stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, true, INV>(x, lds, t, tw);
+    } else {
+        static_assert(R1 * R2 * R3 == M, "radices must multiply to M");
+        stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, false, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R3, R1 * R2, false, true, INV>(x, lds, t, tw);
+    }
+}

@@ -1,0 +1,116 @@
+"""ctypes binding of the C ABI in include/llhip.h (libllhip.so, built in-tree).
+
+This is the same surface a maintainer would bind from the reference side; tests and
+bench.py use it to call the GPU engines without the pybind11 layer.  Loading fails loudly
+if the library has not been built: there is no CPU fallback behind it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libllhip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "llhip.h")
+
+FP64 = 0
+NTT = 1
+ST_OK, ST_ROUNDOFF, ST_NOT_RUN, ST_TRIVIAL = 0, 1, 2, 3
+
+_lib = None
+
+
+def declared_symbols() -> list[str]:
+    """Every llh_* function declared in include/llhip.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(llh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run mersenneprime_python_cpp_gmp_benchmark_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        u64, i32, u32, dbl = ctypes.c_uint64, ctypes.c_int32, ctypes.c_uint32, ctypes.c_double
+        P = ctypes.POINTER
+        L.llh_init.argtypes = [ctypes.c_int]
+        L.llh_last_error.restype = ctypes.c_char_p
+        L.llh_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_int), P(u64),
+                                      ctypes.c_char_p, ctypes.c_size_t]
+        L.llh_fft_length.argtypes = [u64, dbl]
+        L.llh_fft_length.restype = u64
+        L.llh_batch_length.argtypes = [u64, ctypes.c_int]
+        L.llh_batch_length.restype = u64
+        L.llh_ntt_length.argtypes = [u64]
+        L.llh_ntt_length.restype = u64
+        L.llh_max_exponent.argtypes = [ctypes.c_int]
+        L.llh_max_exponent.restype = u64
+        L.llh_set_fft_bits_per_word.argtypes = [dbl]
+        L.llh_get_fft_bits_per_word.restype = dbl
+        L.llh_batch.argtypes = [P(u64), ctypes.c_size_t, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(dbl),
+                                P(u32), P(i32)]
+        L.llh_last_launch_stats.argtypes = [P(dbl), P(u32)]
+        L.llh_lucas_lehmer.argtypes = [u64, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32), P(i32)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(f"llhip error {rc}: {lib().llh_last_error().decode()}")
+
+
+def device_info() -> dict:
+    name = ctypes.create_string_buffer(256)
+    arch = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int(0)
+    mem = ctypes.c_uint64(0)
+    _check(lib().llh_device_info(name, 256, ctypes.byref(cus), ctypes.byref(mem), arch, 64))
+    return {"name": name.value.decode(), "cus": cus.value, "mem": mem.value, "arch": arch.value.decode()}
+
+
+def set_bpw(bits: float) -> None:
+    _check(lib().llh_set_fft_bits_per_word(bits))
+
+
+def get_bpw() -> float:
+    return lib().llh_get_fft_bits_per_word()
+
+
+def batch(ps, iters: int = 0, transform: int = NTT) -> dict:
+    """Run llh_batch; returns numpy arrays verdict, res64, seconds, maxerr, n_used, status."""
+    ps = np.ascontiguousarray(np.asarray(ps, dtype=np.uint64))
+    n = len(ps)
+    out = {
+        "verdict": np.zeros(n, np.int32), "res64": np.zeros(n, np.uint64), "seconds": np.zeros(n, np.float64),
+        "maxerr": np.zeros(n, np.float64), "n_used": np.zeros(n, np.uint32), "status": np.zeros(n, np.int32),
+    }
+    P = ctypes.POINTER
+    c = lambda a, t: a.ctypes.data_as(P(t))  # noqa: E731
+    _check(lib().llh_batch(c(ps, ctypes.c_uint64), n, iters, transform, c(out["verdict"], ctypes.c_int32),
+                           c(out["res64"], ctypes.c_uint64), c(out["seconds"], ctypes.c_double),
+                           c(out["maxerr"], ctypes.c_double), c(out["n_used"], ctypes.c_uint32),
+                           c(out["status"], ctypes.c_int32)))
+    return out
+
+
+def lucas_lehmer(p: int, iters: int = 0, transform: int = NTT) -> dict:
+    v, st = ctypes.c_int32(0), ctypes.c_int32(0)
+    r = ctypes.c_uint64(0)
+    e = ctypes.c_double(0)
+    n = ctypes.c_uint32(0)
+    _check(lib().llh_lucas_lehmer(p, iters, transform, ctypes.byref(v), ctypes.byref(r), ctypes.byref(e),
+                                  ctypes.byref(n), ctypes.byref(st)))
+    return {"verdict": v.value, "res64": r.value, "maxerr": e.value, "n_used": n.value, "status": st.value}
+
+
+def last_launch_stats() -> tuple[float, int]:
+    """(kernel milliseconds, launches) of the last batch call, from HIP events."""
+    ms = ctypes.c_double(0)
+    n = ctypes.c_uint32(0)
+    _check(lib().llh_last_launch_stats(ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value

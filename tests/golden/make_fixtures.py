@@ -1,0 +1,120 @@
+#!/usr/bin/env python3
+"""Writes the committed golden fixtures from the CPU oracle's GMP engine.
+
+The GMP engine restates lucaslehmer.cpp:29-56 (the reference's exact arbiter); Res64 is
+the low 64 bits of the canonical residue.  Each fixture is plain CSV:
+
+  c1_gmp_p10001.csv      every prime 2 <= p <= 10001, full test: p,prime,res64
+  c3_gmp_p100001.csv     every prime 2 <= p <= 100001, full test (hours of CPU)
+  big_gmp_interim.csv    interim residues: p,iters,res64 (p = 1,048,573 @ 10,000;
+                         p = 16,777,213 @ 5,000; p = 3,021,377 @ 20,000 = README.md:200-206)
+  c4_exponents.csv       256 primes from [2^20, 2^25), numpy default_rng(0), with split
+  c4_gmp_heldout.csv     held-out subset: p,iters,res64 at 20,000 iterations
+
+Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp [--threads T]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import oracle as O  # noqa: E402
+
+
+def run_range(hi: int, path: str, threads: int) -> None:
+    ps = O.primes(2, hi)
+    # longest first so the threads finish together; written in ascending order
+    order = sorted(ps, reverse=True)
+    done = {}
+    if os.path.exists(path + ".partial"):
+        for line in open(path + ".partial"):
+            p, v, r = line.strip().split(",")
+            done[int(p)] = (int(v), r)
+    todo = [p for p in order if p not in done]
+    t0 = time.time()
+    chunk = max(threads * 4, 16)
+    with open(path + ".partial", "a") as f:
+        for i in range(0, len(todo), chunk):
+            part = todo[i:i + chunk]
+            out = O.ll_many(part, engine=0, threads=threads)
+            for p, (v, r) in zip(part, out):
+                f.write(f"{p},{int(v)},{r:016x}\n")
+                done[p] = (int(v), f"{r:016x}")
+            f.flush()
+            print(f"{len(done)}/{len(ps)} done, last p={part[-1]}, {time.time() - t0:.0f} s", flush=True)
+    with open(path, "w") as f:
+        f.write("p,prime,res64\n")
+        for p in ps:
+            v, r = done[p]
+            f.write(f"{p},{v},{r}\n")
+    os.remove(path + ".partial")
+
+
+def run_big(path: str) -> None:
+    cases = [(3021377, 20000), (1048573, 10000), (16777213, 5000)]
+    rows = []
+    for p, k in cases:
+        t0 = time.time()
+        v, r, _ = O.ll_gmp(p, k)
+        rows.append((p, k, r))
+        print(f"p={p} k={k} res64={r:016x} {time.time() - t0:.1f} s", flush=True)
+    assert rows[0][2] == 0xCE811E3772129824, "README.md:200-206 interim residue mismatch"
+    with open(path, "w") as f:
+        f.write("p,iters,res64\n")
+        for p, k, r in rows:
+            f.write(f"{p},{k},{r:016x}\n")
+
+
+def c4_list(path: str) -> None:
+    import numpy as np
+    ps = O.primes(1 << 20, (1 << 25) - 1)
+    rng = np.random.default_rng(0)
+    pick = sorted(int(x) for x in rng.choice(np.array(ps, dtype=np.int64), 256, replace=False))
+    # fixed split: every 4th (by ascending p) is held out -> 64 held-out, 192 train
+    with open(path, "w") as f:
+        f.write("p,split\n")
+        for i, p in enumerate(pick):
+            f.write(f"{p},{'heldout' if i % 4 == 3 else 'train'}\n")
+    print(f"{len(ps)} primes in [2^20, 2^25); wrote {len(pick)}")
+
+
+def c4_gmp(list_path: str, path: str, threads: int, count: int) -> None:
+    rows = [l.strip().split(",") for l in open(list_path)][1:]
+    held = [int(p) for p, s in rows if s == "heldout"]
+    # an evenly spread subset of the held-out exponents (GMP at 2^25 bits is ~50 ms/squaring)
+    sub = held[:: max(1, len(held) // count)][:count]
+    t0 = time.time()
+    out = O.ll_many(sub, iters=20000, engine=0, threads=threads)
+    with open(path, "w") as f:
+        f.write("p,iters,res64\n")
+        for p, (_, r) in zip(sub, out):
+            f.write(f"{p},20000,{r:016x}\n")
+    print(f"{len(sub)} held-out residues in {time.time() - t0:.0f} s")
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp"])
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
+    ap.add_argument("--count", type=int, default=8)
+    a = ap.parse_args()
+    if a.what == "c1":
+        run_range(10001, os.path.join(HERE, "c1_gmp_p10001.csv"), a.threads)
+    elif a.what == "c3":
+        run_range(100001, os.path.join(HERE, "c3_gmp_p100001.csv"), a.threads)
+    elif a.what == "big":
+        run_big(os.path.join(HERE, "big_gmp_interim.csv"))
+    elif a.what == "c4list":
+        c4_list(os.path.join(HERE, "c4_exponents.csv"))
+    else:
+        c4_gmp(os.path.join(HERE, "c4_exponents.csv"), os.path.join(HERE, "c4_gmp_heldout.csv"),
+               a.threads, a.count)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,86 @@
+"""Parity of the batched GPU engines with the GMP oracle, through the C ABI.
+
+Bit-exact: every verdict and every Res64 must equal the committed GMP fixtures.
+"""
+import numpy as np
+import pytest
+
+from conftest import KNOWN_100001, load_range_fixture
+from mersenneprime_python_cpp_gmp_benchmark_amd import llhip
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_rows(rows, out):
+    ps = np.array([r[0] for r in rows], dtype=np.uint64)
+    want_v = np.array([r[1] for r in rows])
+    want_r = np.array([r[2] for r in rows], dtype=np.uint64)
+    ok = np.isin(out["status"], [llhip.ST_OK, llhip.ST_TRIVIAL])
+    assert ok.all(), ps[~ok][:10]
+    bad_r = np.nonzero(out["res64"] != want_r)[0]
+    bad_v = np.nonzero((out["verdict"] != 0) != (want_v != 0))[0]
+    assert len(bad_r) == 0, [(int(ps[i]), hex(int(out["res64"][i])), hex(int(want_r[i]))) for i in bad_r[:10]]
+    assert len(bad_v) == 0, ps[bad_v][:10]
+
+
+@pytest.mark.parametrize("transform", [llhip.NTT, llhip.FP64], ids=["ntt", "fp64"])
+def test_c1_all_primes_to_10001_one_launch(transform):
+    rows = load_range_fixture("c1_gmp_p10001.csv")
+    out = llhip.batch([r[0] for r in rows], 0, transform)
+    _check_rows(rows, out)
+    if transform == llhip.FP64:
+        assert float(out["maxerr"].max()) < 0.4
+
+
+@pytest.mark.parametrize("transform", [llhip.NTT, llhip.FP64], ids=["ntt", "fp64"])
+def test_c3_all_primes_to_100001_one_launch(transform):
+    rows = load_range_fixture("c3_gmp_p100001.csv")
+    out = llhip.batch([r[0] for r in rows], 0, transform)
+    _check_rows(rows, out)
+    hits = [int(p) for p, v in zip([r[0] for r in rows], out["verdict"]) if v]
+    assert hits == KNOWN_100001
+
+
+def test_partial_iterations_match_gmp():
+    from oracle import oracle as O
+    rng = np.random.default_rng(7)
+    ps = [int(p) for p in rng.choice(O.primes(65, 30000), 64, replace=False)]
+    ks = [int(rng.integers(1, p - 2)) for p in ps]
+    for tr in (llhip.NTT, llhip.FP64):
+        for p, k in zip(ps[:16], ks[:16]):
+            r = llhip.lucas_lehmer(p, k, tr)
+            assert r["status"] == llhip.ST_OK
+            assert r["res64"] == O.ll_gmp(p, k)[1], (tr, p, k)
+
+
+def test_odd_composite_and_tiny_exponents():
+    from oracle import oracle as O
+    ps = [3, 5, 7, 9, 11, 15, 21, 23, 25, 29, 31, 33, 45, 49, 61, 63] + list(range(65, 1200, 2)) + [4425, 6001]
+    for tr in (llhip.NTT, llhip.FP64):
+        out = llhip.batch(ps, 0, tr)
+        for i, p in enumerate(ps):
+            v, r, _ = O.ll_gmp(p)
+            assert (bool(out["verdict"][i]), int(out["res64"][i])) == (v, r), (tr, p)
+
+
+def test_argument_rules():
+    out = llhip.batch([0, 1, 2, 4, 100], 0, llhip.NTT)
+    assert list(out["verdict"]) == [0, 0, 1, 0, 0]
+    assert all(s == llhip.ST_TRIVIAL for s in out["status"])
+
+
+def test_forced_rollback_stays_exact():
+    """Too many bits per word makes the FP64 round-off exceed 0.4; the engine must roll back
+    to longer lengths and still return GMP's Res64."""
+    rows = load_range_fixture("c1_gmp_p10001.csv")
+    sel = [r for r in rows if r[0] > 3000][::40]
+    old = llhip.get_bpw()
+    try:
+        llhip.set_bpw(40.0)
+        first = np.array([llhip.lib().llh_batch_length(r[0], llhip.FP64) for r in sel])
+        out = llhip.batch([r[0] for r in sel], 0, llhip.FP64)
+    finally:
+        llhip.set_bpw(old)
+    _check_rows(sel, out)
+    assert (out["n_used"] > first).any()          # some exponents were rolled back to 2N
+    assert (out["n_used"] >= first).all()
