@@ -31,15 +31,23 @@
 
 namespace {
 
-constexpr int kWG = 256;
+constexpr int kWG = 512;
 constexpr u32 kNone = 0xFFFFFFFFu;
 constexpr double kErrLimit = 0.4;   // round-off above this aborts the exponent (rollback at 2N)
 
+constexpr int kMaxGroups = 128;
+
 struct Ctl {
-    u32 item[64];
-    u32 bad[64];
-    unsigned long long err[64];
+    u32 item[kMaxGroups];
+    u32 bad[kMaxGroups];
+    unsigned long long err[kMaxGroups];
 };
+
+// LDS of the batch kernels: every group's residue buffer (padded) and the control block.
+// Referenced directly (not through pointer arguments) so every access is a ds_* instruction.
+constexpr int kSmemBytes = kWG * 8 * 16 * 17 / 16;
+__shared__ __attribute__((aligned(16))) char g_smem[kSmemBytes];
+__shared__ Ctl g_ctl;
 
 // Exact canonical residue of N near-balanced digits (natural order, i64, in LDS):
 // full carry propagation with wrap (2^p == 1), then Res64 = low 64 bits of the value in
@@ -79,25 +87,26 @@ __device__ int normalize_res64(i64 *d, u32 p, int log_n, u64 &res, int &zero) {
     return 0;
 }
 
-template <bool IS_GL, int M, int G, int R, int R1, int R2, int R3>
+template <bool IS_GL, int M, int G, int R, int R1, int R2, int R3, int R4>
 __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, const llh::BatchItem *__restrict__ items,
-                          llh::BatchResult *__restrict__ out, u32 *counter, char *smem, Ctl &ctl) {
+                          llh::BatchResult *__restrict__ out, u32 *counter) {
+    Ctl &ctl = g_ctl;
     typedef typename std::conditional<IS_GL, FGL, FC64>::type F;
     typedef typename F::T T;
     typedef typename F::W W;
     typedef typename std::conditional<IS_GL, u64, double>::type WT;
     constexpr int NG = kWG / G;
     constexpr int REG = M + M / 16;
-    static_assert(NG <= 64, "too many groups per workgroup");
+    static_assert(NG <= kMaxGroups, "too many groups per workgroup");
 
     const int tid = threadIdx.x, g = tid / G, t = tid % G;
-    T *lds = reinterpret_cast<T *>(smem) + g * REG;
+    T *lds = reinterpret_cast<T *>(g_smem) + g * REG;
     const int log_n = cd.log_n;
     const u32 N = 1u << log_n, nmask = N - 1;
-    const W *__restrict__ tw = (const W *)cd.tw;
-    const W *__restrict__ twi = IS_GL ? (const W *)cd.twi : tw;
-    const WT *__restrict__ wf = (const WT *)cd.wf;
-    const WT *__restrict__ wi = (const WT *)cd.wi;
+    const gptr<W> tw = as_global<W>(cd.tw);
+    const gptr<W> twi = IS_GL ? as_global<W>(cd.twi) : tw;
+    const gptr<WT> wf = as_global<WT>(cd.wf);
+    const gptr<WT> wi = as_global<WT>(cd.wi);
 
     u32 item = kNone, p = N + 1;
     u64 target = 0, it = 0, t0 = 0;
@@ -141,43 +150,48 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 
         if constexpr (!IS_GL) {
             // ---- weight: z_j = x_{2j} a_{2j} + i x_{2j+1} a_{2j+1}
+            const int tw0 = ll_opaque(t);
+            const u32 pw = ll_opaque(p);
 #pragma unroll
             for (int s = 0; s < R; ++s) {
-                const u32 j = t + G * s;
-                const u32 dp0 = 2u * j * p;
-                const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - p) & nmask;
-                x[s] = cmk(x[s].re * wf[e0], x[s].im * wf[e1]);
+                const u32 j = tw0 + G * s;
+                const u32 dp0 = 2u * j * pw;
+                const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - pw) & nmask;
+                x[s] = cmk(x[s].re * ld_table(wf, e0), x[s].im * ld_table(wf, e1));
             }
-            transform<F, M, G, R, R1, R2, R3, false>(x, lds, t, tw);
+            transform<F, M, G, R, R1, R2, R3, R4, false>(x, lds, t, tw);
             // ---- square the real spectrum in packed form (pairs k, M-k)
+            const int tq = ll_opaque(t);
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < R; ++q) lds[lds_pad(t + G * q)] = x[q];
+            for (int q = 0; q < R; ++q) lds[lds_pad(tq + G * q)] = x[q];
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                const int k = t + G * q;
+                const int k = tq + G * q;
                 const int km = (M - k) & (M - 1);
                 const C64 zm = lds[lds_pad(km)];
                 const C64 D = csub(x[q], cconj(zm));
-                const C64 w = tw[k];
+                const C64 w = ld_table(tw, (u32)k);
                 const C64 A = cmk(1.0 + w.re, w.im);
                 const C64 AD = cmul(A, csqr(D));
                 const C64 Z2 = csqr(x[q]);
                 x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
             }
-            transform<F, M, G, R, R1, R2, R3, true>(x, lds, t, tw);
+            transform<F, M, G, R, R1, R2, R3, R4, true>(x, lds, t, tw);
             // ---- unweight, round, error, -2, carry within the pair
             double cout[R];
+            const int tu = ll_opaque(t);
+            const u32 pu = ll_opaque(p);
 #pragma unroll
             for (int s = 0; s < R; ++s) {
-                const u32 j = t + G * s;
-                const u32 dp0 = 2u * j * p;
-                const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + p + N - 1) >> log_n,
-                          c2 = (dp0 + 2u * p + N - 1) >> log_n;
+                const u32 j = tu + G * s;
+                const u32 dp0 = 2u * j * pu;
+                const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + pu + N - 1) >> log_n,
+                          c2 = (dp0 + 2u * pu + N - 1) >> log_n;
                 const int b0 = (int)(c1 - c0), b1 = (int)(c2 - c1);
-                const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - p) & nmask;
-                const double v0 = x[s].re * wi[e0], v1 = x[s].im * wi[e1];
+                const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - pu) & nmask;
+                const double v0 = x[s].re * ld_table(wi, e0), v1 = x[s].im * ld_table(wi, e1);
                 double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
                 eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
                 if (j == 0 && has) r0 -= 2.0;
@@ -195,17 +209,19 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             // the digits near balanced even at a few bits per digit (one round lets the slack,
             // ~sqrt(N)/4, feed back into the next squaring and grow).
             double *ldsd = reinterpret_cast<double *>(lds);
+            const int tc = ll_opaque(t);
+            const u32 pc = ll_opaque(p);
             __syncthreads();
 #pragma unroll
-            for (int s = 0; s < R; ++s) ldsd[t + G * s] = cout[s];
+            for (int s = 0; s < R; ++s) ldsd[tc + G * s] = cout[s];
             __syncthreads();
 #pragma unroll
             for (int s = 0; s < R; ++s) {
-                const u32 j = t + G * s;
+                const u32 j = tc + G * s;
                 const double cin = ldsd[(j - 1) & (M - 1)];
-                const u32 dp0 = 2u * j * p;
-                const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + p + N - 1) >> log_n,
-                          c2 = (dp0 + 2u * p + N - 1) >> log_n;
+                const u32 dp0 = 2u * j * pc;
+                const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + pc + N - 1) >> log_n,
+                          c2 = (dp0 + 2u * pc + N - 1) >> log_n;
                 const int b0 = (int)(c1 - c0), b1 = (int)(c2 - c1);
                 double x0 = x[s].re + cin;
                 const double cC = floor(__fma_rn(x0, ldexp(1.0, -b0), 0.5));
@@ -218,33 +234,37 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             }
             __syncthreads();
 #pragma unroll
-            for (int s = 0; s < R; ++s) ldsd[t + G * s] = cout[s];
+            for (int s = 0; s < R; ++s) ldsd[tc + G * s] = cout[s];
             __syncthreads();
 #pragma unroll
             for (int s = 0; s < R; ++s) {
-                const u32 j = t + G * s;
+                const u32 j = tc + G * s;
                 x[s].re += ldsd[(j - 1) & (M - 1)];
             }
         } else {
             // ---- Goldilocks: weight, NTT, square, inverse NTT
+            const int tw0 = ll_opaque(t);
+            const u32 pw = ll_opaque(p);
 #pragma unroll
             for (int s = 0; s < R; ++s) {
-                const u32 j = t + G * s;
-                const u32 e = (0u - j * p) & nmask;
-                x[s] = gl_mul(gl_from_i64((i64)x[s]), wf[e]);
+                const u32 j = tw0 + G * s;
+                const u32 e = (0u - j * pw) & nmask;
+                x[s] = gl_mul(gl_from_i64((i64)x[s]), ld_table(wf, e));
             }
-            transform<F, M, G, R, R1, R2, R3, false>(x, lds, t, tw);
+            transform<F, M, G, R, R1, R2, R3, R4, false>(x, lds, t, tw);
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = gl_sqr(x[q]);
-            transform<F, M, G, R, R1, R2, R3, true>(x, lds, t, twi);
+            transform<F, M, G, R, R1, R2, R3, R4, true>(x, lds, t, twi);
             i64 cout[R];
+            const int tu = ll_opaque(t);
+            const u32 pu = ll_opaque(p);
 #pragma unroll
             for (int s = 0; s < R; ++s) {
-                const u32 j = t + G * s;
-                const u32 dp = j * p;
+                const u32 j = tu + G * s;
+                const u32 dp = j * pu;
                 const u32 e = (0u - dp) & nmask;
-                const int b = (int)(((dp + p + N - 1) >> log_n) - ((dp + N - 1) >> log_n));
-                i64 v = gl_to_i64(gl_mul(x[s], wi[e]));
+                const int b = (int)(((dp + pu + N - 1) >> log_n) - ((dp + N - 1) >> log_n));
+                i64 v = gl_to_i64(gl_mul(x[s], ld_table(wi, e)));
                 if (j == 0 && has) v -= 2;
                 const i64 c = (v + ((i64)1 << (b - 1))) >> b;
                 x[s] = (u64)(v - (c << b));
@@ -253,18 +273,20 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             i64 *ldsi = reinterpret_cast<i64 *>(lds);
 #pragma unroll
             for (int round = 0; round < 2; ++round) {
+                const int tc = ll_opaque(t);
+                const u32 pc = ll_opaque(p);
                 __syncthreads();
 #pragma unroll
-                for (int s = 0; s < R; ++s) ldsi[t + G * s] = cout[s];
+                for (int s = 0; s < R; ++s) ldsi[tc + G * s] = cout[s];
                 __syncthreads();
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
-                    const u32 j = t + G * s;
+                    const u32 j = tc + G * s;
                     const i64 cin = ldsi[(j - 1) & nmask];
                     i64 v = (i64)x[s] + cin;
                     if (round == 0) {
-                        const u32 dp = j * p;
-                        const int b = (int)(((dp + p + N - 1) >> log_n) - ((dp + N - 1) >> log_n));
+                        const u32 dp = j * pc;
+                        const int b = (int)(((dp + pc + N - 1) >> log_n) - ((dp + N - 1) >> log_n));
                         const i64 c = (v + ((i64)1 << (b - 1))) >> b;
                         v -= c << b;
                         cout[s] = c;
@@ -368,25 +390,22 @@ __device__ void run_tiny(const llh::ClassDesc &cd, const llh::BatchItem *__restr
 }
 
 template <bool IS_GL>
-__global__ void __launch_bounds__(kWG)
+__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 4)))
 ll_batch_kernel(const llh::ClassDesc *__restrict__ classes, int ncls,
                 const llh::BatchItem *__restrict__ items, llh::BatchResult *__restrict__ out,
                 u32 *counters) {
-    constexpr int kSmem = kWG * 16 * (IS_GL ? 8 : 16) * 17 / 16;
-    __shared__ __attribute__((aligned(16))) char smem[kSmem];
-    __shared__ Ctl ctl;
     for (int c = 0; c < ncls; ++c) {
         const llh::ClassDesc cd = classes[c];
         switch (cd.log_m) {
-            case 12: run_class<IS_GL, 4096, 256, 16, 16, 16, 16>(cd, items, out, counters + c, smem, ctl); break;
-            case 11: run_class<IS_GL, 2048, 128, 16, 16, 8, 16>(cd, items, out, counters + c, smem, ctl); break;
-            case 10: run_class<IS_GL, 1024, 64, 16, 16, 4, 16>(cd, items, out, counters + c, smem, ctl); break;
-            case 9: run_class<IS_GL, 512, 32, 16, 16, 2, 16>(cd, items, out, counters + c, smem, ctl); break;
-            case 8: run_class<IS_GL, 256, 16, 16, 16, 16, 0>(cd, items, out, counters + c, smem, ctl); break;
-            case 7: run_class<IS_GL, 128, 16, 8, 8, 2, 8>(cd, items, out, counters + c, smem, ctl); break;
-            case 6: run_class<IS_GL, 64, 8, 8, 8, 8, 0>(cd, items, out, counters + c, smem, ctl); break;
-            case 5: run_class<IS_GL, 32, 8, 4, 4, 2, 4>(cd, items, out, counters + c, smem, ctl); break;
-            case 4: run_class<IS_GL, 16, 4, 4, 4, 4, 0>(cd, items, out, counters + c, smem, ctl); break;
+            case 12: run_class<IS_GL, 4096, 512, 8, 8, 8, 8, 8>(cd, items, out, counters + c); break;
+            case 11: run_class<IS_GL, 2048, 256, 8, 8, 4, 8, 8>(cd, items, out, counters + c); break;
+            case 10: run_class<IS_GL, 1024, 128, 8, 8, 2, 8, 8>(cd, items, out, counters + c); break;
+            case 9: run_class<IS_GL, 512, 64, 8, 8, 8, 8, 0>(cd, items, out, counters + c); break;
+            case 8: run_class<IS_GL, 256, 32, 8, 8, 4, 8, 0>(cd, items, out, counters + c); break;
+            case 7: run_class<IS_GL, 128, 16, 8, 8, 2, 8, 0>(cd, items, out, counters + c); break;
+            case 6: run_class<IS_GL, 64, 8, 8, 8, 8, 0, 0>(cd, items, out, counters + c); break;
+            case 5: run_class<IS_GL, 32, 8, 4, 4, 2, 4, 0>(cd, items, out, counters + c); break;
+            case 4: run_class<IS_GL, 16, 4, 4, 4, 4, 0, 0>(cd, items, out, counters + c); break;
             case 0: run_tiny(cd, items, out, counters + c); break;
             default: break;
         }

@@ -195,10 +195,10 @@ static int batch_log_n(u64 p, int transform) {
 }
 
 static int groups_per_wg(int log_m) {
-    // G = M / R with R = 16 for M >= 256, R = 8 for M = 64, 128, R = 4 below
+    // 512-thread workgroups; G = M / R threads per exponent, R = 8 (R = 4 for M < 64)
     const int M = 1 << log_m;
-    const int R = M >= 256 ? 16 : (M >= 64 ? 8 : 4);
-    return 256 / (M / R);
+    const int R = M >= 64 ? 8 : 4;
+    return 512 / (M / R);
 }
 
 static int grow(void **ptr, size_t &cap, size_t need) {

@@ -24,6 +24,35 @@ typedef int32_t i32;
 #define LL_DEV __device__ __forceinline__
 #define LL_HD __host__ __device__ __forceinline__
 
+// Returns v unchanged, but opaque to the optimiser.  Index math derived from it is then
+// recomputed where it is used instead of being hoisted out of the squaring loop and kept
+// live in registers for its whole length (twiddle and weight addresses are loop-invariant,
+// and holding them all would cost more VGPRs than the residue itself).
+LL_DEV int ll_opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+LL_DEV u32 ll_opaque(u32 v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Global-memory pointers with their address space spelled out, so loads through them are
+// global_load (32-bit offsets from a uniform base) rather than flat_load of 64-bit addresses.
+#define LL_GLOBAL __attribute__((address_space(1)))
+template <class T>
+using gptr = const T LL_GLOBAL *;
+template <class T>
+LL_DEV gptr<T> as_global(const void *p) {
+    return (gptr<T>)p;
+}
+
+// Load element `idx` of a read-only table.
+template <class T>
+LL_DEV T ld_table(gptr<T> base, u32 idx) {
+    return base[idx];
+}
+
 // ------------------------------------------------------------------------------------------
 // FP64 complex
 // ------------------------------------------------------------------------------------------
@@ -42,6 +71,10 @@ LL_DEV C64 cmulc(C64 a, C64 b) {
 }
 LL_DEV C64 csqr(C64 a) { return cmk(__fma_rn(a.re, a.re, -a.im * a.im), 2.0 * a.re * a.im); }
 LL_DEV C64 cconj(C64 a) { return cmk(a.re, -a.im); }
+LL_DEV C64 ld_table(gptr<C64> base, u32 idx) {
+    gptr<double> d = (gptr<double>)base;
+    return cmk(d[2 * idx], d[2 * idx + 1]);
+}
 
 // ------------------------------------------------------------------------------------------
 // Goldilocks field, q = 2^64 - 2^32 + 1.  Values are kept fully reduced in [0, q).

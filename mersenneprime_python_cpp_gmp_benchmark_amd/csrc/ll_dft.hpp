@@ -94,12 +94,13 @@ LL_DEV void dft_r(typename F::T *x) {
 // FIRST: inputs come from registers;  LAST: outputs stay in registers.
 template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV>
 LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t,
-                          const typename F::W *__restrict__ tw) {
+                          gptr<typename F::W> tw) {
     static_assert(R % r == 0, "radix must divide the per-thread count");
     static_assert(!FIRST || r == R, "first pass must use the full per-thread radix");
     static_assert(!LAST || (r == R && L * r == M), "last pass must use the full radix");
     constexpr int NB = R / r;
     constexpr int STRIDE = M / r;
+    t = ll_opaque(t);
     if constexpr (!FIRST) {
         __syncthreads();
 #pragma unroll
@@ -115,7 +116,7 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t,
             constexpr int TSTEP = M / (r * L);
 #pragma unroll
             for (int s = 1; s < r; ++s)
-                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], tw[s * k * TSTEP]);
+                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], ld_table(tw, (u32)(s * k * TSTEP)));
         }
     }
 #pragma unroll
@@ -132,11 +133,17 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t,
     }
 }
 
-// Whole transform: two or three passes (R3 == 0 means two).
-template <class F, int M, int G, int R, int R1, int R2, int R3, bool INV>
+// Whole transform: two, three or four passes (R3 == 0: two; R4 == 0: three).
+template <class F, int M, int G, int R, int R1, int R2, int R3, int R4, bool INV>
 LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t,
-                      const typename F::W *__restrict__ tw) {
-    if constexpr (R3 == 0) {
+                      gptr<typename F::W> tw) {
+    if constexpr (R4 != 0) {
+        static_assert(R1 * R2 * R3 * R4 == M, "radices must multiply to M");
+        stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, false, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R3, R1 * R2, false, false, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R4, R1 * R2 * R3, false, true, INV>(x, lds, t, tw);
+    } else if constexpr (R3 == 0) {
         static_assert(R1 * R2 == M, "radices must multiply to M");
         stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
         stockham_pass<F, M, G, R, R2, R1, false, true, INV>(x, lds, t, tw);
