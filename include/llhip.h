@@ -79,6 +79,20 @@ int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, i
  * No reference counterpart: measurement hook for the roofline report. */
 int llh_last_launch_stats(double *kernel_ms, uint32_t *launches);
 
+/* One exponent at an explicit transform length n_force (a power of two; 0 = the engine's
+ * length rule).  flags bit 0: no rollback (an FP64 round-off above 0.4 ends the run with
+ * LLH_ST_ROUNDOFF instead of restarting at 2N).  seconds = device time.  No reference
+ * counterpart: the reference fixes N by its size rules (lucaslehmer.cpp:247, cuda/ll_ntt.cu:176);
+ * this is the hook for the learned length choice and for round-off measurements. */
+int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n_force, int flags,
+                       int32_t *verdict, uint64_t *res64, double *maxerr, uint32_t *n_used, int32_t *status,
+                       double *seconds);
+
+/* Per-iteration maximum round-off of the last large-engine FP64 run (one value per squaring);
+ * writes up to cap values and returns how many there are.  The reference only keeps the
+ * maximum over the run (lucaslehmer.cpp:289-295). */
+size_t llh_last_roundoff_trace(double *out, size_t cap);
+
 /* One exponent.  transform = LLH_NTT replaces lucas_lehmer_n lucaslehmer.cpp:29-56 (the exact
  * engine; Res64 of gmp_res64_after :364-365); LLH_FP64 replaces lucas_lehmer_fft_raw
  * lucaslehmer.cpp:256-329 (maxerr of fft_info :337-342 / fft_res64_after :361-363).

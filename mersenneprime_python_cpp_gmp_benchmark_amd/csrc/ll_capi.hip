@@ -24,6 +24,31 @@ hipError_t launch_batch(int transform, const ClassDesc *d_classes, int ncls, con
                         BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream);
 int batch_blocks_per_cu(int transform);
 
+struct LargeArgs {
+    void *elem;
+    void *carry;
+    unsigned long long *err;
+    const void *twW, *twWi;
+    const void *twH, *twHi;
+    const void *tmL, *tmH;
+    const void *tmLi, *tmHi;
+    const void *wfL, *wfH;
+    const void *wiL, *wiH;
+    u32 p, log_n, log_s, log_sw;
+    u32 err_slot;
+};
+struct LargeLaunch {
+    void (*fwd)(LargeArgs);
+    void (*rows)(LargeArgs);
+    void (*inv)(LargeArgs);
+    void (*apply)(LargeArgs);
+    int log_w, log_h;
+};
+bool large_launch_for(int transform, int log_m, LargeLaunch &L);
+hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
+hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
+hipError_t large_normalize(int transform, const void *elem, i64 *dig, u32 p, u32 log_n, void *out, hipStream_t st);
+
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
 
@@ -42,8 +67,17 @@ struct DeviceState {
     size_t cap_items = 0, cap_cls = 0;
 };
 
+struct LargeTables {
+    void *twW = nullptr, *twWi = nullptr, *twH = nullptr, *twHi = nullptr;
+    void *tmL = nullptr, *tmH = nullptr, *tmLi = nullptr, *tmHi = nullptr;
+    void *wfL = nullptr, *wfH = nullptr, *wiL = nullptr, *wiH = nullptr;
+    u32 log_s = 0, log_sw = 0;
+};
+
 static std::recursive_mutex g_mx;
 static DeviceState g_dev;
+static std::map<std::pair<int, int>, LargeTables> g_large_tables;
+static std::vector<double> g_err_trace;     // per-iteration max round-off of the last large run
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
 static std::atomic<double> g_bpw{18.0};
@@ -289,8 +323,239 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
     return LLH_OK;
 }
 
+// ---- large engine ------------------------------------------------------------------------
+static const int kLargeMinLogM = 15, kLargeMaxLogM = 21;
+static const int kGraphBlock = 200;       // squarings per captured graph / round-off check
+
+template <class V>
+static int upload(void **dst, const std::vector<V> &v) {
+    HIPCHK(hipMalloc(dst, v.size() * sizeof(V)));
+    HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    return LLH_OK;
+}
+
+static int get_large_tables(int transform, int log_n, LargeTables &out) {
+    auto key = std::make_pair(transform, log_n);
+    auto it = g_large_tables.find(key);
+    if (it != g_large_tables.end()) {
+        out = it->second;
+        return LLH_OK;
+    }
+    const int log_m = transform == kFP64 ? log_n - 1 : log_n;
+    LargeLaunch L;
+    if (!large_launch_for(transform, log_m, L)) {
+        set_error("no large-engine kernels for this length");
+        return LLH_ERR_UNSUPPORTED;
+    }
+    const u64 M = (u64)1 << log_m, N = (u64)1 << log_n, W = (u64)1 << L.log_w, H = (u64)1 << L.log_h;
+    LargeTables T;
+    T.log_s = (u32)((log_m + 1) / 2);
+    T.log_sw = (u32)((log_n + 1) / 2);
+    const u64 S = (u64)1 << T.log_s, Sw = (u64)1 << T.log_sw;
+    int rc;
+    if (transform == kFP64) {
+        auto roots = [](u64 n, u64 count, u64 stride) {
+            std::vector<double> v(2 * count);
+            for (u64 j = 0; j < count; ++j) {
+                long double c, s;
+                sincos_frac(j * stride, n, c, s);
+                v[2 * j] = (double)c;
+                v[2 * j + 1] = (double)(-s);
+            }
+            return v;
+        };
+        if ((rc = upload(&T.twW, roots(W, W, 1)))) return rc;
+        if ((rc = upload(&T.twH, roots(H, H, 1)))) return rc;
+        if ((rc = upload(&T.tmL, roots(M, S, 1)))) return rc;
+        if ((rc = upload(&T.tmH, roots(M, M / S, S)))) return rc;
+        std::vector<double> wfL(Sw), wfH(N / Sw), wiL(Sw), wiH(N / Sw);
+        for (u64 e = 0; e < Sw; ++e) {
+            long double v = exp2l((long double)e / (long double)N);
+            wfL[e] = (double)v;
+            wiL[e] = (double)(1.0L / v);
+        }
+        for (u64 i = 0; i < N / Sw; ++i) {
+            long double v = exp2l((long double)(i * Sw) / (long double)N);
+            wfH[i] = (double)v;
+            wiH[i] = (double)(1.0L / (v * (long double)M));
+        }
+        if ((rc = upload(&T.wfL, wfL)) || (rc = upload(&T.wfH, wfH)) || (rc = upload(&T.wiL, wiL)) ||
+            (rc = upload(&T.wiH, wiH)))
+            return rc;
+    } else {
+        const u64 w = gl_root_unity_aligned(M), wv = gl_inv_h(w);
+        const u64 r2 = gl_root2_h(N), r2i = gl_inv_h(r2), ninv = gl_inv_h(N);
+        if (!w || !r2) {
+            set_error("no Goldilocks roots for this length");
+            return LLH_ERR_UNSUPPORTED;
+        }
+        auto powers = [](u64 base, u64 count, u64 scale) {
+            std::vector<u64> v(count);
+            u64 a = scale;
+            for (u64 j = 0; j < count; ++j) { v[j] = a; a = gl_mul_h(a, base); }
+            return v;
+        };
+        const u64 wW = gl_pow_h(w, H), wH = gl_pow_h(w, W);   // w_W = w_M^H, w_H = w_M^W
+        if ((rc = upload(&T.twW, powers(wW, W, 1))) || (rc = upload(&T.twWi, powers(gl_inv_h(wW), W, 1))) ||
+            (rc = upload(&T.twH, powers(wH, H, 1))) || (rc = upload(&T.twHi, powers(gl_inv_h(wH), H, 1))) ||
+            (rc = upload(&T.tmL, powers(w, S, 1))) || (rc = upload(&T.tmH, powers(gl_pow_h(w, S), M / S, 1))) ||
+            (rc = upload(&T.tmLi, powers(wv, S, 1))) || (rc = upload(&T.tmHi, powers(gl_pow_h(wv, S), M / S, 1))) ||
+            (rc = upload(&T.wfL, powers(r2, Sw, 1))) || (rc = upload(&T.wfH, powers(gl_pow_h(r2, Sw), N / Sw, 1))) ||
+            (rc = upload(&T.wiL, powers(r2i, Sw, 1))) ||
+            (rc = upload(&T.wiH, powers(gl_pow_h(r2i, Sw), N / Sw, ninv))))
+            return rc;
+    }
+    g_large_tables[key] = T;
+    out = T;
+    return LLH_OK;
+}
+
+struct NormOutH {
+    unsigned long long res64;
+    int zero;
+    int status;
+};
+
+// One exponent on the large engine at length 2^log_n.  Round-off is checked after every
+// block of kGraphBlock squarings; above 0.4 the run stops with kStRoundoff.
+static int run_large(u64 p, u64 iters, int transform, int log_n, BatchResult &r) {
+    const int log_m = transform == kFP64 ? log_n - 1 : log_n;
+    LargeLaunch L;
+    if (log_m < kLargeMinLogM || log_m > kLargeMaxLogM || !large_launch_for(transform, log_m, L)) {
+        r.status = kStNotRun;
+        return LLH_OK;
+    }
+    LargeTables T;
+    int rc = get_large_tables(transform, log_n, T);
+    if (rc) return rc;
+    const u64 M = (u64)1 << log_m, N = (u64)1 << log_n, H = (u64)1 << L.log_h, W = (u64)1 << L.log_w;
+    const size_t esz = transform == kFP64 ? 16 : 8;
+    const u64 nblk = W / 4;
+    void *elem = nullptr, *carry = nullptr, *dig = nullptr, *errs = nullptr, *nout = nullptr;
+    HIPCHK(hipMalloc(&elem, M * esz));
+    HIPCHK(hipMalloc(&carry, nblk * H * 8));
+    HIPCHK(hipMalloc(&dig, N * 8));
+    HIPCHK(hipMalloc(&errs, kGraphBlock * 8));
+    HIPCHK(hipMalloc(&nout, sizeof(NormOutH)));
+    hipStream_t st = g_dev.stream;
+    HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
+    HIPCHK(hipMemsetAsync(carry, 0, nblk * H * 8, st));
+    if (transform == kFP64) {
+        const double four = 4.0;
+        HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+    } else {
+        const u64 four = 4;
+        HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+    }
+    LargeArgs A;
+    A.elem = elem; A.carry = carry; A.err = (unsigned long long *)errs;
+    A.twW = T.twW; A.twWi = T.twWi; A.twH = T.twH; A.twHi = T.twHi;
+    A.tmL = T.tmL; A.tmH = T.tmH; A.tmLi = T.tmLi; A.tmHi = T.tmHi;
+    A.wfL = T.wfL; A.wfH = T.wfH; A.wiL = T.wiL; A.wiH = T.wiH;
+    A.p = (u32)p; A.log_n = (u32)log_n; A.log_s = T.log_s; A.log_sw = T.log_sw; A.err_slot = 0;
+
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<double> eh(kGraphBlock);
+    g_err_trace.clear();
+    g_err_trace.reserve(iters);
+    hipEvent_t ev0, ev1;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventRecord(ev0, st));
+    r.status = kStOk;
+    r.maxerr = 0;
+    u64 done = 0;
+    while (done < iters) {
+        const u64 k = std::min<u64>(kGraphBlock, iters - done);
+        HIPCHK(hipMemsetAsync(errs, 0, kGraphBlock * 8, st));
+        if (k == (u64)kGraphBlock) {
+            if (!gexec) {
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                for (u64 i = 0; i < k; ++i) {
+                    A.err_slot = (u32)i;
+                    hipError_t e = large_step(L, A, st);
+                    if (e != hipSuccess) {
+                        hipGraph_t g2;
+                        (void)hipStreamEndCapture(st, &g2);
+                        set_error(std::string("HIP: ") + hipGetErrorString(e) + " while capturing");
+                        return LLH_ERR_HIP;
+                    }
+                }
+                HIPCHK(hipStreamEndCapture(st, &graph));
+                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(gexec, st));
+        } else {
+            for (u64 i = 0; i < k; ++i) {
+                A.err_slot = (u32)i;
+                HIPCHK(large_step(L, A, st));
+            }
+        }
+        HIPCHK(hipMemcpyAsync(eh.data(), errs, kGraphBlock * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        double bmax = 0;
+        for (u64 i = 0; i < k; ++i) {
+            g_err_trace.push_back(eh[i]);
+            bmax = std::max(bmax, eh[i]);
+        }
+        r.maxerr = std::max(r.maxerr, bmax);
+        done += k;
+        if (transform == kFP64 && bmax > 0.4) {
+            r.status = kStRoundoff;
+            break;
+        }
+    }
+    NormOutH no{0, 0, 0};
+    if (r.status == kStOk) {
+        HIPCHK(large_apply_carry(L, A, st));
+        HIPCHK(large_normalize(transform, elem, (i64 *)dig, (u32)p, (u32)log_n, nout, st));
+        HIPCHK(hipMemcpyAsync(&no, nout, sizeof(no), hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipEventRecord(ev1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    g_last_kernel_ms += ms;
+    g_last_launches += 1;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    r.iters_done = done;
+    r.t0 = 0;
+    r.t1 = (u64)((double)ms * 1e-3 * g_dev.wclk_hz);
+    if (r.status == kStOk) {
+        if (no.status) r.status = kStNotRun;
+        r.res64 = no.res64;
+        r.verdict = no.zero;
+    }
+    (void)hipFree(elem);
+    (void)hipFree(carry);
+    (void)hipFree(dig);
+    (void)hipFree(errs);
+    (void)hipFree(nout);
+    return LLH_OK;
+}
+
+// log2 of the large engine's length for p (0 = none): smallest power of two with p/N <= bpw
+static int large_log_n(u64 p, int transform) {
+    const int lo = transform == kFP64 ? kLargeMinLogM + 1 : kLargeMinLogM;
+    const int hi = transform == kFP64 ? kLargeMaxLogM + 1 : kLargeMaxLogM;
+    for (int l = lo; l <= hi; ++l) {
+        const u64 N = (u64)1 << l;
+        if (transform == kFP64) {
+            if ((double)p / (double)N <= g_bpw.load()) return l;
+        } else if ((p + N - 1) / N <= (u64)ntt_max_bits(N)) {
+            return l;
+        }
+    }
+    return 0;
+}
+
 static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int32_t *verdict,
-                      uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status) {
+                      uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status,
+                      int force_log_n = 0, bool no_rollback = false) {
     if (transform != kFP64 && transform != kNTT) {
         set_error("transform must be LLH_FP64 or LLH_NTT");
         return LLH_ERR_ARG;
@@ -304,7 +569,7 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
     if (rc) return rc;
     g_last_kernel_ms = 0.0;
     g_last_launches = 0;
-    std::vector<Pending> work;
+    std::vector<Pending> work, large;
     std::vector<double> secs(n, 0.0), errs(n, 0.0);
     for (size_t i = 0; i < n; ++i) {
         const u64 p = ps[i];
@@ -317,9 +582,43 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
         if (p % 2 == 0) continue;                              // :32
         if (p >= ((u64)1 << 31)) { status[i] = kStNotRun; continue; }
         const u64 it = max_iters ? std::min<u64>(max_iters, p - 2) : p - 2;
+        const int batch_max = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
+        const int batch_min = transform == kFP64 ? kFp64MinLog : kNttMinLog;
+        if (force_log_n && p >= 64) {
+            if (force_log_n >= batch_min && force_log_n <= batch_max &&
+                (((u64)1 << force_log_n) + 2) * p < ((u64)1 << 32))
+                work.push_back(Pending{p, it, i, force_log_n});
+            else
+                large.push_back(Pending{p, it, i, force_log_n});
+            continue;
+        }
         const int ln = batch_log_n(p, transform);
-        if (p >= 64 && ln == 0) { status[i] = kStNotRun; continue; }
+        if (p >= 64 && ln == 0) {
+            large.push_back(Pending{p, it, i, large_log_n(p, transform)});
+            continue;
+        }
         work.push_back(Pending{p, it, i, ln});
+    }
+    // exponents beyond the batched engine: one at a time on the large engine (longest first)
+    for (const Pending &w : large) {
+        BatchResult r = BatchResult();
+        if (w.log_n == 0) { status[w.idx] = kStNotRun; continue; }
+        int log_n = w.log_n;
+        for (;;) {
+            rc = run_large(w.p, w.iters, transform, log_n, r);
+            if (rc) return rc;
+            secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
+            errs[w.idx] = std::max(errs[w.idx], r.maxerr);
+            if (r.status == kStRoundoff && transform == kFP64 && log_n < kLargeMaxLogM + 1 && !no_rollback) {
+                ++log_n;                                   // rollback: redo at 2N
+                continue;
+            }
+            break;
+        }
+        verdict[w.idx] = r.verdict;
+        res64[w.idx] = r.res64;
+        status[w.idx] = r.status;
+        if (n_used) n_used[w.idx] = 1u << log_n;
     }
     const int max_log = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
     while (!work.empty()) {
@@ -332,7 +631,7 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
             const BatchResult &r = res[k];
             secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
             errs[w.idx] = std::max(errs[w.idx], r.maxerr);
-            if (r.status == kStRoundoff && w.log_n < max_log &&
+            if (r.status == kStRoundoff && w.log_n < max_log && !no_rollback &&
                 (((u64)2 << w.log_n) + 2) * w.p < ((u64)1 << 32)) {
                 again.push_back(Pending{w.p, w.iters, w.idx, w.log_n + 1});   // rollback at 2N
                 continue;
@@ -420,6 +719,46 @@ int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, i
         set_error(e.what());
         return LLH_ERR_HIP;
     }
+}
+
+int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n_force, int flags,
+                       int32_t *verdict, uint64_t *res64, double *maxerr, uint32_t *n_used, int32_t *status,
+                       double *seconds) {
+    int log_n = 0;
+    if (n_force) {
+        while (((uint64_t)1 << log_n) < n_force) ++log_n;
+        if (((uint64_t)1 << log_n) != n_force || log_n < 4 || log_n > 24) {
+            set_error("n_force must be a power of two in [16, 2^24]");
+            return LLH_ERR_ARG;
+        }
+    }
+    double sec = 0, err = 0;
+    uint32_t nu = 0;
+    int32_t v = 0, st = 0;
+    uint64_t r = 0;
+    int rc;
+    try {
+        rc = batch_impl(&p, 1, max_iters, transform, &v, &r, &sec, &err, &nu, &st, log_n, (flags & 1) != 0);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return LLH_ERR_HIP;
+    }
+    if (rc) return rc;
+    if (verdict) *verdict = v;
+    if (res64) *res64 = r;
+    if (maxerr) *maxerr = err;
+    if (n_used) *n_used = nu;
+    if (status) *status = st;
+    if (seconds) *seconds = sec;
+    return LLH_OK;
+}
+
+size_t llh_last_roundoff_trace(double *out, size_t cap) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    const size_t n = g_err_trace.size();
+    if (out)
+        for (size_t i = 0; i < n && i < cap; ++i) out[i] = g_err_trace[i];
+    return n;
 }
 
 int llh_last_launch_stats(double *kernel_ms, uint32_t *launches) {

@@ -1,0 +1,531 @@
+// ll_large.hip -- Lucas-Lehmer for one large exponent: the residue lives in HBM and each
+// squaring is three kernel launches of a four-step transform.
+//
+// Counterpart of the FP64 loop lucaslehmer.cpp:281-311 (and of the NTT loop of
+// cuda/ll_ntt.cu:91-163, which the reference only runs up to N = 8192) for transform lengths
+// 2^15 .. 2^22 elements.  Layout: the M transform elements form an H x W matrix, element
+// j = w + W h (row h, column w; rows contiguous).  With k = k_h + H k_w:
+//
+//   Z[k] = sum_w w_W^{w k_w} w_M^{w k_h} sum_h z[w + W h] w_H^{h k_h}
+//
+//  K1 cols_fwd  (blocks of CW columns): carry-in of the previous squaring, weights,
+//               H-point FFT per column, twiddle w_M^{w k_h}; in place.
+//  K2 rows      (rows k_h and H - k_h together): W-point FFT per row, square (FP64: the
+//               packed real spectrum pairs row k_h with row H - k_h), inverse W-point FFT.
+//  K3 cols_inv  (blocks of CW columns): twiddle w_M^{-w k_h}, inverse H-point FFT, unweight,
+//               round, per-iteration max round-off, -2, and an exact balanced carry along each
+//               row segment of CW elements; the segment's carry-out goes to a carry buffer
+//               and K1 of the next squaring adds it to the next segment (the last segment of
+//               row h feeds row h + 1, and the top one wraps to digit 0: 2^p == 1).
+//
+// Digits: FP64 packs digits 2j, 2j+1 into element j (N = 2M); GL holds digit j in element j
+// (N = M).  Weights and the w_M twiddles come from two-level tables (product of a low and
+// a high table entry), so every table stays cache resident at every length.
+#include "ll_common.hpp"
+#include "ll_dft.hpp"
+#include "ll_internal.hpp"
+#include <type_traits>
+
+namespace {
+
+constexpr int kCW = 4;       // columns per column-kernel workgroup
+constexpr int kR = 8;        // elements per thread in every large-engine FFT
+
+// radix schedule of a 2^K-point transform with 8 elements per thread (first = last = 8)
+template <int K>
+struct Sched {
+    static_assert(K >= 6 && K <= 11, "transform length out of range");
+    static constexpr int R1 = 8;
+    static constexpr int R2 = K == 6 ? 8 : K == 7 ? 2 : K == 8 ? 4 : K == 9 ? 8 : K == 10 ? 2 : 4;
+    static constexpr int R3 = K == 6 ? 0 : K <= 9 ? 8 : 8;
+    static constexpr int R4 = K <= 9 ? 0 : 8;
+};
+
+template <bool IS_GL>
+struct ElemOf {
+    typedef typename std::conditional<IS_GL, FGL, FC64>::type F;
+    typedef typename F::T T;
+    typedef typename std::conditional<IS_GL, u64, double>::type WT;   // weight table type
+    typedef typename std::conditional<IS_GL, i64, double>::type CT;   // carry type
+};
+
+}  // namespace
+
+namespace llh {
+
+// Kernel arguments (by value).  Table pointers are device addresses.
+struct LargeArgs {
+    void *elem;              // [M] elements
+    void *carry;             // [nblk][H] row-segment carry-outs (double or i64)
+    unsigned long long *err; // per-iteration max round-off slots (double bits)
+    const void *twW, *twWi;  // w_W^j, j < W   (GL: forward, inverse)
+    const void *twH, *twHi;  // w_H^j, j < H
+    const void *tmL, *tmH;   // w_M^e = tmL[e & (S-1)] * tmH[e >> s]
+    const void *tmLi, *tmHi; // GL inverse of the above
+    const void *wfL, *wfH;   // forward weight of exponent e: wfL[e & (Sw-1)] * wfH[e >> sw]
+    const void *wiL, *wiH;   // inverse weight (with the transform's 1/M or 1/N)
+    u32 p, log_n, log_s, log_sw;
+    u32 err_slot;
+};
+
+}  // namespace llh
+
+namespace {
+
+using llh::LargeArgs;
+
+template <class T>
+LL_DEV T tw2(const void *lo, const void *hi, u32 e, u32 log_s);
+
+template <>
+LL_DEV C64 tw2<C64>(const void *lo, const void *hi, u32 e, u32 log_s) {
+    return cmul(ld_table(as_global<C64>(lo), e & ((1u << log_s) - 1)), ld_table(as_global<C64>(hi), e >> log_s));
+}
+template <>
+LL_DEV u64 tw2<u64>(const void *lo, const void *hi, u32 e, u32 log_s) {
+    return gl_mul(ld_table(as_global<u64>(lo), e & ((1u << log_s) - 1)), ld_table(as_global<u64>(hi), e >> log_s));
+}
+LL_DEV double wt2d(const void *lo, const void *hi, u32 e, u32 log_s) {
+    return ld_table(as_global<double>(lo), e & ((1u << log_s) - 1)) * ld_table(as_global<double>(hi), e >> log_s);
+}
+
+// c_d = ceil(d p / N) for power-of-two N
+LL_DEV u64 cbit(u64 d, u32 p, u32 log_n) { return (d * p + ((1ull << log_n) - 1)) >> log_n; }
+LL_DEV int bits_of(u64 d, u32 p, u32 log_n) { return (int)(cbit(d + 1, p, log_n) - cbit(d, p, log_n)); }
+LL_DEV u32 wexp(u64 d, u32 p, u32 log_n) { return (0u - (u32)(d * p)) & ((1u << log_n) - 1); }
+
+// balanced split of v at b bits: v = x + c 2^b, x in [-2^(b-1), 2^(b-1))
+LL_DEV double split_d(double v, int b, double &c) {
+    c = floor(__fma_rn(v, ldexp(1.0, -b), 0.5));
+    return __fma_rn(-c, ldexp(1.0, b), v);
+}
+LL_DEV i64 split_i(i64 v, int b, i64 &c) {
+    c = (v + ((i64)1 << (b - 1))) >> b;
+    return v - (c << b);
+}
+
+// ---- K1 -----------------------------------------------------------------------------------
+template <bool IS_GL, int LOG_W, int LOG_H>
+__global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_fwd(LargeArgs A) {
+    typedef ElemOf<IS_GL> E;
+    typedef typename E::F F;
+    typedef typename E::T T;
+    typedef typename E::CT CT;
+    typedef Sched<LOG_H> S;
+    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, GH = H / kR, NT = kCW * GH;
+    constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
+    constexpr int NBLK = W / kCW;
+    __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
+    const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
+    const int blk = blockIdx.x, w = blk * kCW + c;
+    T *lds = lds_all + c * REG;
+    T *elem = reinterpret_cast<T *>(A.elem);
+    const CT *carry = reinterpret_cast<const CT *>(A.carry);
+    const u32 p = A.p, log_n = A.log_n;
+
+    T x[kR];
+#pragma unroll
+    for (int s = 0; s < kR; ++s) x[s] = elem[w + W * (t + GH * s)];
+    // ---- carry-in: transpose to rows, add the previous segment's carry-out, re-split
+#pragma unroll
+    for (int s = 0; s < kR; ++s) lds_all[(t + GH * s) * kCW + c] = x[s];
+    __syncthreads();
+    for (int h = tid; h < H; h += NT) {
+        CT cin;
+        if (blk > 0) cin = carry[(blk - 1) * H + h];
+        else cin = h > 0 ? carry[(NBLK - 1) * H + h - 1] : carry[(NBLK - 1) * H + H - 1];
+        T *row = lds_all + h * kCW;
+        const u64 j0 = (u64)blk * kCW + (u64)W * h;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) {
+            if constexpr (IS_GL) {
+                const int b = bits_of(j0 + cc, p, log_n);
+                i64 v = (i64)row[cc] + cin;
+                if (cc == kCW - 1) { row[cc] = (u64)v; break; }
+                i64 cc2;
+                v = split_i(v, b, cc2);
+                row[cc] = (u64)v;
+                cin = cc2;
+            } else {
+                const u64 d0 = 2 * (j0 + cc);
+                const int b0 = bits_of(d0, p, log_n), b1 = bits_of(d0 + 1, p, log_n);
+                double c1, c2;
+                const double x0 = split_d(row[cc].re + cin, b0, c1);
+                double x1 = row[cc].im + c1;
+                if (cc == kCW - 1) { row[cc] = cmk(x0, x1); break; }
+                x1 = split_d(x1, b1, c2);
+                row[cc] = cmk(x0, x1);
+                cin = c2;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kR; ++s) x[s] = lds_all[(t + GH * s) * kCW + c];
+    // ---- weights
+    {
+        const int tt = ll_opaque(t);
+#pragma unroll
+        for (int s = 0; s < kR; ++s) {
+            const u64 j = (u64)w + (u64)W * (tt + GH * s);
+            if constexpr (IS_GL) {
+                const u32 e = wexp(j, p, log_n);
+                x[s] = gl_mul(gl_from_i64((i64)x[s]), tw2<u64>(A.wfL, A.wfH, e, A.log_sw));
+            } else {
+                const u32 e0 = wexp(2 * j, p, log_n), e1 = wexp(2 * j + 1, p, log_n);
+                x[s] = cmk(x[s].re * wt2d(A.wfL, A.wfH, e0, A.log_sw), x[s].im * wt2d(A.wfL, A.wfH, e1, A.log_sw));
+            }
+        }
+    }
+    transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, as_global<T>(A.twH));
+    // ---- twiddle w_M^{w k_h}, k_h = t + GH q, and store
+    {
+        const int tt = ll_opaque(t);
+#pragma unroll
+        for (int q = 0; q < kR; ++q) {
+            const u32 kh = tt + GH * q;
+            const u32 e = (u32)w * kh;
+            if constexpr (IS_GL) x[q] = gl_mul(x[q], tw2<u64>(A.tmL, A.tmH, e, A.log_s));
+            else x[q] = cmul(x[q], tw2<C64>(A.tmL, A.tmH, e, A.log_s));
+            elem[w + W * kh] = x[q];
+        }
+    }
+}
+
+// ---- K2 -----------------------------------------------------------------------------------
+template <bool IS_GL, int LOG_W, int LOG_H>
+__global__ void __launch_bounds__(2 * ((1 << LOG_W) / kR)) lg_rows(LargeArgs A) {
+    typedef ElemOf<IS_GL> E;
+    typedef typename E::F F;
+    typedef typename E::T T;
+    typedef Sched<LOG_W> S;
+    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, GW = W / kR;
+    constexpr int REG = W + W / 16 + 4;
+    __shared__ __attribute__((aligned(16))) T lds_all[2 * REG];
+    const int tid = threadIdx.x, g = tid / GW, t = tid % GW;
+    const int i = blockIdx.x;
+    const int row = g == 0 ? i : (i == 0 ? H / 2 : H - i);
+    T *lds = lds_all + g * REG;
+    T *elem = reinterpret_cast<T *>(A.elem) + (size_t)W * row;
+
+    T x[kR];
+#pragma unroll
+    for (int s = 0; s < kR; ++s) x[s] = elem[t + GW * s];
+    transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, as_global<T>(A.twW));
+    if constexpr (IS_GL) {
+#pragma unroll
+        for (int q = 0; q < kR; ++q) x[q] = gl_sqr(x[q]);
+        transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, as_global<T>(A.twWi));
+    } else {
+        const int tt = ll_opaque(t);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kR; ++q) lds[lds_pad(tt + GW * q)] = x[q];
+        __syncthreads();
+        // partner of (row, k_w): row 0 -> (0, -k_w); row H/2 -> (H/2, W-1-k_w);
+        // otherwise -> (H - row, W-1-k_w), held by the other group of this workgroup
+        const T *plds = (row == 0 || row == H / 2) ? lds : lds_all + (1 - g) * REG;
+#pragma unroll
+        for (int q = 0; q < kR; ++q) {
+            const int kw = tt + GW * q;
+            const int kwp = row == 0 ? ((W - kw) & (W - 1)) : (W - 1 - kw);
+            const C64 zm = plds[lds_pad(kwp)];
+            const C64 D = csub(x[q], cconj(zm));
+            const u32 k = (u32)row + (u32)H * (u32)kw;
+            const C64 wk = tw2<C64>(A.tmL, A.tmH, k, A.log_s);
+            const C64 AD = cmul(cmk(1.0 + wk.re, wk.im), csqr(D));
+            const C64 Z2 = csqr(x[q]);
+            x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
+        }
+        transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, as_global<T>(A.twW));
+    }
+#pragma unroll
+    for (int s = 0; s < kR; ++s) elem[t + GW * s] = x[s];
+}
+
+// ---- K3 -----------------------------------------------------------------------------------
+template <bool IS_GL, int LOG_W, int LOG_H>
+__global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_inv(LargeArgs A) {
+    typedef ElemOf<IS_GL> E;
+    typedef typename E::F F;
+    typedef typename E::T T;
+    typedef typename E::CT CT;
+    typedef Sched<LOG_H> S;
+    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, GH = H / kR, NT = kCW * GH;
+    constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
+    __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
+    __shared__ unsigned long long s_err;
+    const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
+    const int blk = blockIdx.x, w = blk * kCW + c;
+    T *lds = lds_all + c * REG;
+    T *elem = reinterpret_cast<T *>(A.elem);
+    CT *carry = reinterpret_cast<CT *>(A.carry);
+    const u32 p = A.p, log_n = A.log_n;
+    if (tid == 0) s_err = 0;
+
+    T x[kR];
+    {
+        const int tt = ll_opaque(t);
+#pragma unroll
+        for (int s = 0; s < kR; ++s) {
+            const u32 kh = tt + GH * s;
+            const u32 e = (u32)w * kh;
+            x[s] = elem[w + W * kh];
+            if constexpr (IS_GL) x[s] = gl_mul(x[s], tw2<u64>(A.tmLi, A.tmHi, e, A.log_s));
+            else x[s] = cmulc(x[s], tw2<C64>(A.tmL, A.tmH, e, A.log_s));
+        }
+    }
+    transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, true>(
+        x, lds, t, as_global<T>(IS_GL ? A.twHi : A.twH));
+    // ---- unweight and round (rows h = t + GH q)
+    double eloc = 0.0;
+    {
+        const int tt = ll_opaque(t);
+#pragma unroll
+        for (int q = 0; q < kR; ++q) {
+            const u64 j = (u64)w + (u64)W * (tt + GH * q);
+            if constexpr (IS_GL) {
+                const u32 e = wexp(j, p, log_n);
+                i64 v = gl_to_i64(gl_mul(x[q], tw2<u64>(A.wiL, A.wiH, e, A.log_sw)));
+                if (j == 0) v -= 2;
+                x[q] = (u64)v;
+            } else {
+                const u32 e0 = wexp(2 * j, p, log_n), e1 = wexp(2 * j + 1, p, log_n);
+                const double v0 = x[q].re * wt2d(A.wiL, A.wiH, e0, A.log_sw);
+                const double v1 = x[q].im * wt2d(A.wiL, A.wiH, e1, A.log_sw);
+                double r0 = __builtin_rint(v0);
+                const double r1 = __builtin_rint(v1);
+                eloc = fmax(eloc, fmax(fabs(v0 - r0), fabs(v1 - r1)));
+                if (j == 0) r0 -= 2.0;
+                x[q] = cmk(r0, r1);
+            }
+        }
+    }
+    // ---- balanced carry along each row segment of kCW elements
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kR; ++q) lds_all[(t + GH * q) * kCW + c] = x[q];
+    __syncthreads();
+    for (int h = tid; h < H; h += NT) {
+        T *row = lds_all + h * kCW;
+        const u64 j0 = (u64)blk * kCW + (u64)W * h;
+        CT cy = 0;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) {
+            if constexpr (IS_GL) {
+                i64 c2;
+                const i64 v = split_i((i64)row[cc] + cy, bits_of(j0 + cc, p, log_n), c2);
+                row[cc] = (u64)v;
+                cy = c2;
+            } else {
+                const u64 d0 = 2 * (j0 + cc);
+                double c1, c2;
+                const double x0 = split_d(row[cc].re + cy, bits_of(d0, p, log_n), c1);
+                const double x1 = split_d(row[cc].im + c1, bits_of(d0 + 1, p, log_n), c2);
+                row[cc] = cmk(x0, x1);
+                cy = c2;
+            }
+        }
+        carry[blk * H + h] = cy;
+        // the segment is contiguous in memory
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) elem[blk * kCW + cc + (size_t)W * h] = row[cc];
+    }
+    if constexpr (!IS_GL) {
+        atomicMax(&s_err, (unsigned long long)__double_as_longlong(eloc));
+        __syncthreads();
+        if (tid == 0) atomicMax(A.err + A.err_slot, s_err);
+    }
+}
+
+// ---- carry-in only (after the last squaring, before normalisation) ----------------------------
+template <bool IS_GL, int LOG_W, int LOG_H>
+__global__ void __launch_bounds__(256) lg_apply_carry(LargeArgs A) {
+    typedef ElemOf<IS_GL> E;
+    typedef typename E::T T;
+    typedef typename E::CT CT;
+    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, NBLK = W / kCW;
+    const int blk = blockIdx.x;
+    T *elem = reinterpret_cast<T *>(A.elem);
+    const CT *carry = reinterpret_cast<const CT *>(A.carry);
+    const u32 p = A.p, log_n = A.log_n;
+    for (int h = threadIdx.x; h < H; h += 256) {
+        CT cin;
+        if (blk > 0) cin = carry[(blk - 1) * H + h];
+        else cin = h > 0 ? carry[(NBLK - 1) * H + h - 1] : carry[(NBLK - 1) * H + H - 1];
+        T *row = elem + blk * kCW + (size_t)W * h;
+        const u64 j0 = (u64)blk * kCW + (u64)W * h;
+        for (int cc = 0; cc < kCW; ++cc) {
+            if constexpr (IS_GL) {
+                i64 v = (i64)row[cc] + cin;
+                if (cc == kCW - 1) { row[cc] = (u64)v; break; }
+                i64 c2;
+                v = split_i(v, bits_of(j0 + cc, p, log_n), c2);
+                row[cc] = (u64)v;
+                cin = c2;
+            } else {
+                const u64 d0 = 2 * (j0 + cc);
+                double c1, c2;
+                const double x0 = split_d(row[cc].re + cin, bits_of(d0, p, log_n), c1);
+                double x1 = row[cc].im + c1;
+                if (cc == kCW - 1) { row[cc] = cmk(x0, x1); break; }
+                x1 = split_d(x1, bits_of(d0 + 1, p, log_n), c2);
+                row[cc] = cmk(x0, x1);
+                cin = c2;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---- exact normalisation of N digits (one workgroup) --------------------------------------------
+// Digits (near balanced) are read as i64 from the element array; each thread normalises a
+// contiguous segment to non-negative digits; carries then ripple between segments in rounds
+// (usually one digit deep) until none is left, the top segment's carry wrapping to digit 0.
+// Output: res64 of the canonical residue and whether it is zero (the all-ones pattern
+// M_p counts as zero).  Restates the GMP rebuild of lucaslehmer.cpp:312-323 on the device.
+namespace {
+constexpr int kNormThreads = 1024;
+
+struct NormOut {
+    unsigned long long res64;
+    int zero;
+    int status;
+};
+
+template <bool IS_GL>
+__global__ void __launch_bounds__(kNormThreads) lg_normalize(const void *elem_v, i64 *dig, u32 p, u32 log_n,
+                                                           NormOut *out) {
+    const u64 N = 1ull << log_n;
+    const u64 K = (N + kNormThreads - 1) / kNormThreads;
+    const int t = threadIdx.x;
+    const u64 d0 = (u64)t * K, d1 = d0 + K < N ? d0 + K : N;
+    __shared__ i64 s_cout[kNormThreads];
+    __shared__ int s_any, s_ones, s_zero;
+    // gather digits to the i64 scratch array
+    for (u64 d = d0; d < d1; ++d) {
+        if constexpr (IS_GL) dig[d] = (i64)reinterpret_cast<const u64 *>(elem_v)[d];
+        else dig[d] = (i64)reinterpret_cast<const double *>(elem_v)[d];
+    }
+    // local normalisation to non-negative digits
+    i64 cy = 0;
+    for (u64 d = d0; d < d1; ++d) {
+        const int b = bits_of(d, p, log_n);
+        const i64 v = dig[d] + cy;
+        dig[d] = v & (((i64)1 << b) - 1);
+        cy = v >> b;
+    }
+    s_cout[t] = cy;
+    __syncthreads();
+    for (int round = 0; round < 4 * kNormThreads + 8; ++round) {
+        __syncthreads();              // every thread has read the previous round's flag
+        if (t == 0) s_any = 0;
+        __syncthreads();
+        i64 cin = s_cout[(t + kNormThreads - 1) % kNormThreads];
+        __syncthreads();
+        cy = 0;
+        if (cin != 0 && d0 < d1) {
+            for (u64 d = d0; d < d1 && cin != 0; ++d) {
+                const int b = bits_of(d, p, log_n);
+                const i64 v = dig[d] + cin;
+                dig[d] = v & (((i64)1 << b) - 1);
+                cin = v >> b;
+            }
+            cy = cin;
+        }
+        s_cout[t] = cy;
+        if (cy != 0) s_any = 1;
+        __syncthreads();
+        if (!s_any) break;
+    }
+    if (t == 0) { s_ones = 1; s_zero = 1; }
+    __syncthreads();
+    int ones = 1, zero = 1;
+    for (u64 d = d0; d < d1; ++d) {
+        const u64 v = (u64)dig[d];
+        if (v) zero = 0;
+        if (v != ((1ull << bits_of(d, p, log_n)) - 1)) ones = 0;
+    }
+    if (!ones) s_ones = 0;
+    if (!zero) s_zero = 0;
+    __syncthreads();
+    if (t == 0) {
+        u64 r = 0;
+        for (u64 d = 0; d < N; ++d) {
+            const u64 cd = cbit(d, p, log_n);
+            if (cd >= 64) break;
+            r |= (u64)dig[d] << cd;
+        }
+        const bool z = s_zero || s_ones;
+        out->res64 = z ? 0 : r;
+        out->zero = z;
+        out->status = s_any ? 1 : 0;
+    }
+}
+
+}  // namespace
+
+namespace llh {
+
+struct LargeLaunch {
+    void (*fwd)(LargeArgs);
+    void (*rows)(LargeArgs);
+    void (*inv)(LargeArgs);
+    void (*apply)(LargeArgs);
+    int log_w, log_h;
+};
+
+template <bool IS_GL, int LW, int LH>
+static LargeLaunch make_launch() {
+    return LargeLaunch{lg_cols_fwd<IS_GL, LW, LH>, lg_rows<IS_GL, LW, LH>, lg_cols_inv<IS_GL, LW, LH>,
+                       lg_apply_carry<IS_GL, LW, LH>, LW, LH};
+}
+
+// log2 of the element count M -> (W, H) split and kernels
+bool large_launch_for(int transform, int log_m, LargeLaunch &L) {
+    const bool gl = transform == kNTT;
+#define LL_CASE(LM, LW, LH)                                              \
+    case LM:                                                             \
+        L = gl ? make_launch<true, LW, LH>() : make_launch<false, LW, LH>(); \
+        return true;
+    switch (log_m) {
+        LL_CASE(15, 8, 7)
+        LL_CASE(16, 8, 8)
+        LL_CASE(17, 9, 8)
+        LL_CASE(18, 9, 9)
+        LL_CASE(19, 10, 9)
+        LL_CASE(20, 10, 10)
+        LL_CASE(21, 11, 10)
+        default: return false;
+    }
+#undef LL_CASE
+}
+
+hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st) {
+    const int W = 1 << L.log_w, H = 1 << L.log_h;
+    const dim3 gcol(W / kCW), bcol(kCW * (H / kR));
+    const dim3 grow(H / 2), brow(2 * (W / kR));
+    hipLaunchKernelGGL(L.fwd, gcol, bcol, 0, st, A);
+    hipLaunchKernelGGL(L.rows, grow, brow, 0, st, A);
+    hipLaunchKernelGGL(L.inv, gcol, bcol, 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st) {
+    hipLaunchKernelGGL(L.apply, dim3((1 << L.log_w) / kCW), dim3(256), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t large_normalize(int transform, const void *elem, i64 *dig, u32 p, u32 log_n, void *out,
+                           hipStream_t st) {
+    if (transform == kNTT)
+        hipLaunchKernelGGL(lg_normalize<true>, dim3(1), dim3(kNormThreads), 0, st, elem, dig, p, log_n,
+                           (NormOut *)out);
+    else
+        hipLaunchKernelGGL(lg_normalize<false>, dim3(1), dim3(kNormThreads), 0, st, elem, dig, p, log_n,
+                           (NormOut *)out);
+    return hipGetLastError();
+}
+
+}  // namespace llh

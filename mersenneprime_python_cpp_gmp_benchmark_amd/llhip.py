@@ -54,6 +54,10 @@ def lib():
         L.llh_batch.argtypes = [P(u64), ctypes.c_size_t, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(dbl),
                                 P(u32), P(i32)]
         L.llh_last_launch_stats.argtypes = [P(dbl), P(u32)]
+        L.llh_lucas_lehmer_n.argtypes = [u64, u64, ctypes.c_int, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32),
+                                         P(i32), P(dbl)]
+        L.llh_last_roundoff_trace.argtypes = [P(dbl), ctypes.c_size_t]
+        L.llh_last_roundoff_trace.restype = ctypes.c_size_t
         L.llh_lucas_lehmer.argtypes = [u64, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32), P(i32)]
         _lib = L
     return _lib
@@ -114,3 +118,23 @@ def last_launch_stats() -> tuple[float, int]:
     n = ctypes.c_uint32(0)
     _check(lib().llh_last_launch_stats(ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+def lucas_lehmer_n(p: int, iters: int = 0, transform: int = FP64, n: int = 0, no_rollback: bool = False) -> dict:
+    """One exponent at an explicit transform length n (0 = the engine's rule)."""
+    v, st = ctypes.c_int32(0), ctypes.c_int32(0)
+    r = ctypes.c_uint64(0)
+    e, sec = ctypes.c_double(0), ctypes.c_double(0)
+    nu = ctypes.c_uint32(0)
+    _check(lib().llh_lucas_lehmer_n(p, iters, transform, n, 1 if no_rollback else 0, ctypes.byref(v), ctypes.byref(r),
+                                    ctypes.byref(e), ctypes.byref(nu), ctypes.byref(st), ctypes.byref(sec)))
+    return {"verdict": v.value, "res64": r.value, "maxerr": e.value, "n_used": nu.value, "status": st.value,
+            "seconds": sec.value}
+
+
+def roundoff_trace() -> np.ndarray:
+    """Per-iteration max round-off of the last large-engine FP64 run."""
+    n = lib().llh_last_roundoff_trace(None, 0)
+    out = np.zeros(n, np.float64)
+    lib().llh_last_roundoff_trace(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+    return out

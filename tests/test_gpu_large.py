@@ -1,0 +1,80 @@
+"""Parity of the large-exponent engine (residue in HBM, three launches per squaring) with
+the GMP oracle, through the C ABI: interim Res64 at every supported length, both transforms,
+full tests of known Mersenne primes, and the committed interim fixtures."""
+import pytest
+
+from conftest import load_interim_fixture
+from mersenneprime_python_cpp_gmp_benchmark_amd import llhip
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# (transform, N, p, iterations): p sized to 8..19 bits per digit for FP64, <= the exact
+# bound for the NTT
+CASES = [
+    (llhip.FP64, 1 << 16, 1048573, 300),
+    (llhip.FP64, 1 << 16, 786433 + 2, 200),
+    (llhip.FP64, 1 << 17, 2000003, 120),
+    (llhip.FP64, 1 << 18, 4194301, 80),
+    (llhip.FP64, 1 << 19, 8388593, 40),
+    (llhip.FP64, 1 << 20, 16777213, 25),
+    (llhip.FP64, 1 << 21, 33554393, 12),
+    (llhip.NTT, 1 << 15, 500009, 300),
+    (llhip.NTT, 1 << 16, 1048573, 200),
+    (llhip.NTT, 1 << 17, 2000003, 100),
+    (llhip.NTT, 1 << 18, 4194301, 60),
+    (llhip.NTT, 1 << 19, 8388593, 30),
+    (llhip.NTT, 1 << 20, 16777213, 20),
+    (llhip.NTT, 1 << 21, 33554393, 10),
+]
+
+
+@pytest.mark.parametrize("tr,n,p,k", CASES, ids=[f"{'ntt' if c[0] else 'fp64'}-N{c[1]}-p{c[2]}" for c in CASES])
+def test_interim_res64_matches_gmp(tr, n, p, k):
+    r = llhip.lucas_lehmer_n(p, k, tr, n)
+    assert r["status"] == llhip.ST_OK, r
+    assert r["n_used"] == n
+    assert r["res64"] == O.ll_gmp(p, k)[1]
+    if tr == llhip.FP64:
+        assert r["maxerr"] < 0.4
+        tr_ = llhip.roundoff_trace()
+        assert len(tr_) == k and abs(tr_.max() - r["maxerr"]) < 1e-12
+
+
+@pytest.mark.parametrize("tr,n", [(llhip.FP64, 1 << 16), (llhip.NTT, 1 << 15)], ids=["fp64", "ntt"])
+def test_known_prime_full_test(tr, n):
+    # M216091 is a Mersenne prime (known_mp_list.txt): verdict True, Res64 0
+    r = llhip.lucas_lehmer_n(216091, 0, tr, n)
+    assert r["status"] == llhip.ST_OK and r["verdict"] == 1 and r["res64"] == 0, r
+
+
+def test_known_prime_full_test_756839():
+    r = llhip.lucas_lehmer_n(756839, 0, llhip.FP64, 1 << 16)
+    assert r["status"] == llhip.ST_OK and r["verdict"] == 1 and r["res64"] == 0, r
+
+
+def test_composite_full_test_matches_gmp():
+    p = 216071          # prime exponent, M_p composite
+    want = O.ll_gmp(p)
+    for tr, n in ((llhip.FP64, 1 << 16), (llhip.NTT, 1 << 15)):
+        r = llhip.lucas_lehmer_n(p, 0, tr, n)
+        assert (bool(r["verdict"]), r["res64"]) == want[:2]
+
+
+def test_interim_fixtures():
+    for p, k, res in load_interim_fixture("big_gmp_interim.csv"):
+        if p > 4_000_000:
+            continue           # the p = 2^24 case runs in the bench
+        for tr in (llhip.FP64, llhip.NTT):
+            r = llhip.lucas_lehmer(p, k, tr)
+            assert r["status"] == llhip.ST_OK and r["res64"] == res, (tr, p, k, r)
+
+
+def test_default_length_routes_large_exponents():
+    # exponents beyond the batched engine go to the large engine inside one llh_batch call
+    ps = [99991, 1048573, 2000003]
+    for tr in (llhip.FP64, llhip.NTT):
+        out = llhip.batch(ps, 50, tr)
+        for i, p in enumerate(ps):
+            assert out["status"][i] == llhip.ST_OK
+            assert int(out["res64"][i]) == O.ll_gmp(p, 50)[1]
