@@ -79,14 +79,20 @@ int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, i
  * No reference counterpart: measurement hook for the roofline report. */
 int llh_last_launch_stats(double *kernel_ms, uint32_t *launches);
 
-/* One exponent at an explicit transform length n_force (a power of two; 0 = the engine's
- * length rule).  flags bit 0: no rollback (an FP64 round-off above 0.4 ends the run with
+/* One exponent at an explicit transform length n_force (one of llh_supported_lengths; 0 = the
+ * engine's length rule).  flags bit 0: no rollback (an FP64 round-off above 0.4 ends the run with
  * LLH_ST_ROUNDOFF instead of restarting at 2N).  seconds = device time.  No reference
  * counterpart: the reference fixes N by its size rules (lucaslehmer.cpp:247, cuda/ll_ntt.cu:176);
  * this is the hook for the learned length choice and for round-off measurements. */
 int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n_force, int flags,
                        int32_t *verdict, uint64_t *res64, double *maxerr, uint32_t *n_used, int32_t *status,
                        double *seconds);
+
+/* Transform lengths (digit counts N) the engines run for `transform`, ascending: the batched
+ * engine's powers of two and the large engine's m * 2^k (FP64: m in {1, 3, 5, 7, 15}).
+ * Writes up to cap values, returns the count.  Counterpart of the reference's length ladder
+ * {1, 3, 5} * 2^k (lucaslehmer.cpp:250-252). */
+size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap);
 
 /* Per-iteration maximum round-off of the last large-engine FP64 run (one value per squaring);
  * writes up to cap values and returns how many there are.  The reference only keeps the

@@ -36,6 +36,8 @@ struct LargeArgs {
     const void *wiL, *wiH;
     u32 p, log_n, log_s, log_sw;
     u32 err_slot;
+    u64 n;
+    double inv_n;
 };
 struct LargeLaunch {
     void (*fwd)(LargeArgs);
@@ -44,10 +46,10 @@ struct LargeLaunch {
     void (*apply)(LargeArgs);
     int log_w, log_h;
 };
-bool large_launch_for(int transform, int log_m, LargeLaunch &L);
+bool large_launch_for(int transform, u64 M, LargeLaunch &L);
 hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
 hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
-hipError_t large_normalize(int transform, const void *elem, i64 *dig, u32 p, u32 log_n, void *out, hipStream_t st);
+hipError_t large_normalize(int transform, const void *elem, i64 *dig, const LargeArgs &A, void *out, hipStream_t st);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -76,7 +78,7 @@ struct LargeTables {
 
 static std::recursive_mutex g_mx;
 static DeviceState g_dev;
-static std::map<std::pair<int, int>, LargeTables> g_large_tables;
+static std::map<std::pair<int, u64>, LargeTables> g_large_tables;
 static std::vector<double> g_err_trace;     // per-iteration max round-off of the last large run
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
@@ -249,7 +251,8 @@ struct Pending {
     u64 p;
     u64 iters;
     size_t idx;
-    int log_n;
+    int log_n;          // batched engine: log2 of the length
+    u64 n = 0;          // large engine: the length
 };
 
 // One persistent launch over `work`; fills res[i] for work[i].
@@ -324,7 +327,6 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
 }
 
 // ---- large engine ------------------------------------------------------------------------
-static const int kLargeMinLogM = 15, kLargeMaxLogM = 21;
 static const int kGraphBlock = 200;       // squarings per captured graph / round-off check
 
 template <class V>
@@ -334,24 +336,31 @@ static int upload(void **dst, const std::vector<V> &v) {
     return LLH_OK;
 }
 
-static int get_large_tables(int transform, int log_n, LargeTables &out) {
-    auto key = std::make_pair(transform, log_n);
+static int ceil_log2(u64 v) {
+    int l = 0;
+    while (((u64)1 << l) < v) ++l;
+    return l;
+}
+
+static int get_large_tables(int transform, u64 N, LargeTables &out) {
+    auto key = std::make_pair(transform, N);
     auto it = g_large_tables.find(key);
     if (it != g_large_tables.end()) {
         out = it->second;
         return LLH_OK;
     }
-    const int log_m = transform == kFP64 ? log_n - 1 : log_n;
+    const u64 M = transform == kFP64 ? N / 2 : N;
     LargeLaunch L;
-    if (!large_launch_for(transform, log_m, L)) {
+    if (!large_launch_for(transform, M, L)) {
         set_error("no large-engine kernels for this length");
         return LLH_ERR_UNSUPPORTED;
     }
-    const u64 M = (u64)1 << log_m, N = (u64)1 << log_n, W = (u64)1 << L.log_w, H = (u64)1 << L.log_h;
+    const u64 W = (u64)1 << L.log_w, H = (u64)L.log_h;
     LargeTables T;
-    T.log_s = (u32)((log_m + 1) / 2);
-    T.log_sw = (u32)((log_n + 1) / 2);
+    T.log_s = (u32)((ceil_log2(M) + 1) / 2);
+    T.log_sw = (u32)((ceil_log2(N) + 1) / 2);
     const u64 S = (u64)1 << T.log_s, Sw = (u64)1 << T.log_sw;
+    const u64 nS = (M + S - 1) / S, nSw = (N + Sw - 1) / Sw;
     int rc;
     if (transform == kFP64) {
         auto roots = [](u64 n, u64 count, u64 stride) {
@@ -367,14 +376,14 @@ static int get_large_tables(int transform, int log_n, LargeTables &out) {
         if ((rc = upload(&T.twW, roots(W, W, 1)))) return rc;
         if ((rc = upload(&T.twH, roots(H, H, 1)))) return rc;
         if ((rc = upload(&T.tmL, roots(M, S, 1)))) return rc;
-        if ((rc = upload(&T.tmH, roots(M, M / S, S)))) return rc;
-        std::vector<double> wfL(Sw), wfH(N / Sw), wiL(Sw), wiH(N / Sw);
+        if ((rc = upload(&T.tmH, roots(M, nS, S)))) return rc;
+        std::vector<double> wfL(Sw), wfH(nSw), wiL(Sw), wiH(nSw);
         for (u64 e = 0; e < Sw; ++e) {
             long double v = exp2l((long double)e / (long double)N);
             wfL[e] = (double)v;
             wiL[e] = (double)(1.0L / v);
         }
-        for (u64 i = 0; i < N / Sw; ++i) {
+        for (u64 i = 0; i < nSw; ++i) {
             long double v = exp2l((long double)(i * Sw) / (long double)N);
             wfH[i] = (double)v;
             wiH[i] = (double)(1.0L / (v * (long double)M));
@@ -398,11 +407,11 @@ static int get_large_tables(int transform, int log_n, LargeTables &out) {
         const u64 wW = gl_pow_h(w, H), wH = gl_pow_h(w, W);   // w_W = w_M^H, w_H = w_M^W
         if ((rc = upload(&T.twW, powers(wW, W, 1))) || (rc = upload(&T.twWi, powers(gl_inv_h(wW), W, 1))) ||
             (rc = upload(&T.twH, powers(wH, H, 1))) || (rc = upload(&T.twHi, powers(gl_inv_h(wH), H, 1))) ||
-            (rc = upload(&T.tmL, powers(w, S, 1))) || (rc = upload(&T.tmH, powers(gl_pow_h(w, S), M / S, 1))) ||
-            (rc = upload(&T.tmLi, powers(wv, S, 1))) || (rc = upload(&T.tmHi, powers(gl_pow_h(wv, S), M / S, 1))) ||
-            (rc = upload(&T.wfL, powers(r2, Sw, 1))) || (rc = upload(&T.wfH, powers(gl_pow_h(r2, Sw), N / Sw, 1))) ||
+            (rc = upload(&T.tmL, powers(w, S, 1))) || (rc = upload(&T.tmH, powers(gl_pow_h(w, S), nS, 1))) ||
+            (rc = upload(&T.tmLi, powers(wv, S, 1))) || (rc = upload(&T.tmHi, powers(gl_pow_h(wv, S), nS, 1))) ||
+            (rc = upload(&T.wfL, powers(r2, Sw, 1))) || (rc = upload(&T.wfH, powers(gl_pow_h(r2, Sw), nSw, 1))) ||
             (rc = upload(&T.wiL, powers(r2i, Sw, 1))) ||
-            (rc = upload(&T.wiH, powers(gl_pow_h(r2i, Sw), N / Sw, ninv))))
+            (rc = upload(&T.wiH, powers(gl_pow_h(r2i, Sw), nSw, ninv))))
             return rc;
     }
     g_large_tables[key] = T;
@@ -418,17 +427,18 @@ struct NormOutH {
 
 // One exponent on the large engine at length 2^log_n.  Round-off is checked after every
 // block of kGraphBlock squarings; above 0.4 the run stops with kStRoundoff.
-static int run_large(u64 p, u64 iters, int transform, int log_n, BatchResult &r) {
-    const int log_m = transform == kFP64 ? log_n - 1 : log_n;
+static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
+    const u64 M = transform == kFP64 ? N / 2 : N;
     LargeLaunch L;
-    if (log_m < kLargeMinLogM || log_m > kLargeMaxLogM || !large_launch_for(transform, log_m, L)) {
+    if (!N || !large_launch_for(transform, M, L) || (N + 2) * (double)p >= 9.0e15) {
         r.status = kStNotRun;
         return LLH_OK;
     }
     LargeTables T;
-    int rc = get_large_tables(transform, log_n, T);
+    int rc = get_large_tables(transform, N, T);
     if (rc) return rc;
-    const u64 M = (u64)1 << log_m, N = (u64)1 << log_n, H = (u64)1 << L.log_h, W = (u64)1 << L.log_w;
+    const u64 H = (u64)L.log_h, W = (u64)1 << L.log_w;
+    const int log_n = (N & (N - 1)) ? 0 : ceil_log2(N);
     const size_t esz = transform == kFP64 ? 16 : 8;
     const u64 nblk = W / 4;
     void *elem = nullptr, *carry = nullptr, *dig = nullptr, *errs = nullptr, *nout = nullptr;
@@ -453,6 +463,7 @@ static int run_large(u64 p, u64 iters, int transform, int log_n, BatchResult &r)
     A.tmL = T.tmL; A.tmH = T.tmH; A.tmLi = T.tmLi; A.tmHi = T.tmHi;
     A.wfL = T.wfL; A.wfH = T.wfH; A.wiL = T.wiL; A.wiH = T.wiH;
     A.p = (u32)p; A.log_n = (u32)log_n; A.log_s = T.log_s; A.log_sw = T.log_sw; A.err_slot = 0;
+    A.n = N; A.inv_n = 1.0 / (double)N;
 
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -509,7 +520,7 @@ static int run_large(u64 p, u64 iters, int transform, int log_n, BatchResult &r)
     NormOutH no{0, 0, 0};
     if (r.status == kStOk) {
         HIPCHK(large_apply_carry(L, A, st));
-        HIPCHK(large_normalize(transform, elem, (i64 *)dig, (u32)p, (u32)log_n, nout, st));
+        HIPCHK(large_normalize(transform, elem, (i64 *)dig, A, nout, st));
         HIPCHK(hipMemcpyAsync(&no, nout, sizeof(no), hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipEventRecord(ev1, st));
@@ -538,24 +549,45 @@ static int run_large(u64 p, u64 iters, int transform, int log_n, BatchResult &r)
     return LLH_OK;
 }
 
-// log2 of the large engine's length for p (0 = none): smallest power of two with p/N <= bpw
-static int large_log_n(u64 p, int transform) {
-    const int lo = transform == kFP64 ? kLargeMinLogM + 1 : kLargeMinLogM;
-    const int hi = transform == kFP64 ? kLargeMaxLogM + 1 : kLargeMaxLogM;
-    for (int l = lo; l <= hi; ++l) {
-        const u64 N = (u64)1 << l;
-        if (transform == kFP64) {
-            if ((double)p / (double)N <= g_bpw.load()) return l;
-        } else if ((p + N - 1) / N <= (u64)ntt_max_bits(N)) {
-            return l;
+// Lengths the large engine runs, ascending.  FP64: {1, 3, 5, 7, 15} * 2^k for N in
+// [2^15, 2^22]; GL: powers of two 2^15 .. 2^21.
+static std::vector<u64> large_lengths(int transform) {
+    std::vector<u64> v;
+    LargeLaunch L;
+    for (u64 m : {1ull, 3ull, 5ull, 7ull, 15ull})
+        for (int k = 10; k <= 22; ++k) {
+            const u64 N = m << k;
+            const u64 M = transform == kFP64 ? N / 2 : N;
+            if (large_launch_for(transform, M, L)) v.push_back(N);
         }
+    std::sort(v.begin(), v.end());
+    return v;
+}
+
+// Default large-engine length for p.  FP64: the reference's conservative rule
+// (lucaslehmer.cpp:244-254: smallest {1,3,5} * 2^k with p/N <= bits-per-word cap);
+// GL: the smallest power of two within the exactness bound (cuda/ll_ntt.cu:172-175).
+static u64 large_length(u64 p, int transform) {
+    if (transform == kFP64) {
+        const u64 N = fft_length_conservative(p, g_bpw.load());
+        LargeLaunch L;
+        return large_launch_for(transform, N / 2, L) ? N : 0;
     }
+    for (u64 N : large_lengths(transform))
+        if ((p + N - 1) / N <= (u64)ntt_max_bits(N)) return N;
+    return 0;
+}
+
+// the next supported length above N (0 = none): the rollback target
+static u64 next_large_length(u64 N, int transform) {
+    for (u64 n : large_lengths(transform))
+        if (n > N) return n;
     return 0;
 }
 
 static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int32_t *verdict,
                       uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status,
-                      int force_log_n = 0, bool no_rollback = false) {
+                      u64 force_n = 0, bool no_rollback = false) {
     if (transform != kFP64 && transform != kNTT) {
         set_error("transform must be LLH_FP64 or LLH_NTT");
         return LLH_ERR_ARG;
@@ -584,17 +616,17 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
         const u64 it = max_iters ? std::min<u64>(max_iters, p - 2) : p - 2;
         const int batch_max = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
         const int batch_min = transform == kFP64 ? kFp64MinLog : kNttMinLog;
-        if (force_log_n && p >= 64) {
-            if (force_log_n >= batch_min && force_log_n <= batch_max &&
-                (((u64)1 << force_log_n) + 2) * p < ((u64)1 << 32))
-                work.push_back(Pending{p, it, i, force_log_n});
+        if (force_n && p >= 64) {
+            const int fl = (force_n & (force_n - 1)) ? 0 : ceil_log2(force_n);
+            if (fl >= batch_min && fl <= batch_max && (force_n + 2) * p < ((u64)1 << 32))
+                work.push_back(Pending{p, it, i, fl});
             else
-                large.push_back(Pending{p, it, i, force_log_n});
+                large.push_back(Pending{p, it, i, 0, force_n});
             continue;
         }
         const int ln = batch_log_n(p, transform);
         if (p >= 64 && ln == 0) {
-            large.push_back(Pending{p, it, i, large_log_n(p, transform)});
+            large.push_back(Pending{p, it, i, 0, large_length(p, transform)});
             continue;
         }
         work.push_back(Pending{p, it, i, ln});
@@ -602,23 +634,23 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
     // exponents beyond the batched engine: one at a time on the large engine (longest first)
     for (const Pending &w : large) {
         BatchResult r = BatchResult();
-        if (w.log_n == 0) { status[w.idx] = kStNotRun; continue; }
-        int log_n = w.log_n;
+        if (w.n == 0) { status[w.idx] = kStNotRun; continue; }
+        u64 N = w.n;
         for (;;) {
-            rc = run_large(w.p, w.iters, transform, log_n, r);
+            rc = run_large(w.p, w.iters, transform, N, r);
             if (rc) return rc;
             secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
             errs[w.idx] = std::max(errs[w.idx], r.maxerr);
-            if (r.status == kStRoundoff && transform == kFP64 && log_n < kLargeMaxLogM + 1 && !no_rollback) {
-                ++log_n;                                   // rollback: redo at 2N
-                continue;
+            if (r.status == kStRoundoff && transform == kFP64 && !no_rollback) {
+                const u64 nx = next_large_length(N, transform);   // rollback: redo longer
+                if (nx) { N = nx; continue; }
             }
             break;
         }
         verdict[w.idx] = r.verdict;
         res64[w.idx] = r.res64;
         status[w.idx] = r.status;
-        if (n_used) n_used[w.idx] = 1u << log_n;
+        if (n_used) n_used[w.idx] = (uint32_t)N;
     }
     const int max_log = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
     while (!work.empty()) {
@@ -724,13 +756,9 @@ int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, i
 int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n_force, int flags,
                        int32_t *verdict, uint64_t *res64, double *maxerr, uint32_t *n_used, int32_t *status,
                        double *seconds) {
-    int log_n = 0;
-    if (n_force) {
-        while (((uint64_t)1 << log_n) < n_force) ++log_n;
-        if (((uint64_t)1 << log_n) != n_force || log_n < 4 || log_n > 24) {
-            set_error("n_force must be a power of two in [16, 2^24]");
-            return LLH_ERR_ARG;
-        }
+    if (n_force && (n_force < 16 || n_force > ((uint64_t)1 << 24))) {
+        set_error("n_force must be in [16, 2^24]");
+        return LLH_ERR_ARG;
     }
     double sec = 0, err = 0;
     uint32_t nu = 0;
@@ -738,7 +766,7 @@ int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n
     uint64_t r = 0;
     int rc;
     try {
-        rc = batch_impl(&p, 1, max_iters, transform, &v, &r, &sec, &err, &nu, &st, log_n, (flags & 1) != 0);
+        rc = batch_impl(&p, 1, max_iters, transform, &v, &r, &sec, &err, &nu, &st, n_force, (flags & 1) != 0);
     } catch (const std::exception &e) {
         set_error(e.what());
         return LLH_ERR_HIP;
@@ -751,6 +779,19 @@ int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n
     if (status) *status = st;
     if (seconds) *seconds = sec;
     return LLH_OK;
+}
+
+size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap) {
+    std::vector<u64> v;
+    const int lo = transform == kFP64 ? kFp64MinLog : kNttMinLog;
+    const int hi = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
+    for (int l = lo; l <= hi; ++l) v.push_back((u64)1 << l);
+    for (u64 n : large_lengths(transform))
+        if (std::find(v.begin(), v.end(), n) == v.end()) v.push_back(n);
+    std::sort(v.begin(), v.end());
+    if (out)
+        for (size_t i = 0; i < v.size() && i < cap; ++i) out[i] = v[i];
+    return v.size();
 }
 
 size_t llh_last_roundoff_trace(double *out, size_t cap) {

@@ -14,6 +14,7 @@
 // middle hand-offs go through LDS.  LDS indices are padded by one element per sixteen so
 // the stride-16 stores of a radix-16 pass do not pile onto one bank.
 #pragma once
+#include <type_traits>
 #include "ll_common.hpp"
 
 LL_HD constexpr int lds_pad(int i) { return i + (i >> 4); }
@@ -37,11 +38,76 @@ LL_DEV void dft4(typename F::T &a0, typename F::T &a1, typename F::T &a2, typena
     a3 = F::sub(t1, t3);
 }
 
+// Odd-length DFT of FP64 complex data (r = 3, 5, 7), symmetric form: with
+// a_s = x_s + x_{r-s}, b_s = x_s - x_{r-s} (s = 1 .. (r-1)/2) and theta = 2 pi s q / r,
+//   y_q = x_0 + sum a_s cos(theta) - i sum b_s sin(theta),   y_{r-q} = the same with +i
+// (the inverse transform swaps the two).
+template <int r>
+struct OddRoots {
+    // cos / sin of 2 pi j / r, j < r (plain constants, long double literals rounded once)
+    static constexpr double c(int j) {
+        return r == 3 ? (j % 3 == 0 ? 1.0 : -0.5)
+             : r == 5 ? (j % 5 == 0 ? 1.0 : (j % 5 == 1 || j % 5 == 4) ? 0.309016994374947424102293417183 : -0.809016994374947424102293417183)
+             : (j % 7 == 0 ? 1.0 : (j % 7 == 1 || j % 7 == 6) ? 0.623489801858733530525004884004
+                               : (j % 7 == 2 || j % 7 == 5) ? -0.222520933956314404288902564497
+                                                            : -0.900968867902419126236102319507);
+    }
+    static constexpr double s(int j) {
+        return r == 3 ? (j % 3 == 0 ? 0.0 : (j % 3 == 1 ? 0.866025403784438646763723170753 : -0.866025403784438646763723170753))
+             : r == 5 ? (j % 5 == 0 ? 0.0 : j % 5 == 1 ? 0.951056516295153572116439333379 : j % 5 == 2 ? 0.587785252292473129168705954639
+                        : j % 5 == 3 ? -0.587785252292473129168705954639 : -0.951056516295153572116439333379)
+             : (j % 7 == 0 ? 0.0 : j % 7 == 1 ? 0.781831482468029808708444526674 : j % 7 == 2 ? 0.974927912181823607018131682994
+                : j % 7 == 3 ? 0.433883739117558120475768332849 : j % 7 == 4 ? -0.433883739117558120475768332849
+                : j % 7 == 5 ? -0.974927912181823607018131682994 : -0.781831482468029808708444526674);
+    }
+};
+
+template <int r, bool INV>
+LL_DEV void dft_odd(C64 *x) {
+    constexpr int h = (r - 1) / 2;
+    C64 a[h], b[h];
+#pragma unroll
+    for (int s = 1; s <= h; ++s) {
+        a[s - 1] = cadd(x[s], x[r - s]);
+        b[s - 1] = csub(x[s], x[r - s]);
+    }
+    C64 y[r];
+    C64 y0 = x[0];
+#pragma unroll
+    for (int s = 0; s < h; ++s) y0 = cadd(y0, a[s]);
+    y[0] = y0;
+#pragma unroll
+    for (int q = 1; q <= h; ++q) {
+        C64 A = x[0], B = cmk(0.0, 0.0);
+#pragma unroll
+        for (int s = 1; s <= h; ++s) {
+            constexpr int dummy = 0;
+            (void)dummy;
+            const double cs = OddRoots<r>::c(s * q), sn = OddRoots<r>::s(s * q);
+            A = cmk(__fma_rn(a[s - 1].re, cs, A.re), __fma_rn(a[s - 1].im, cs, A.im));
+            B = cmk(__fma_rn(b[s - 1].re, sn, B.re), __fma_rn(b[s - 1].im, sn, B.im));
+        }
+        // -i B = (B.im, -B.re)
+        if constexpr (!INV) {
+            y[q] = cmk(A.re + B.im, A.im - B.re);
+            y[r - q] = cmk(A.re - B.im, A.im + B.re);
+        } else {
+            y[q] = cmk(A.re - B.im, A.im + B.re);
+            y[r - q] = cmk(A.re + B.im, A.im - B.re);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < r; ++q) x[q] = y[q];
+}
+
 // y[q] = sum_s x[s] w_r^{s q} in place, natural order in and out.
 template <class F, int r, bool INV>
 LL_DEV void dft_r(typename F::T *x) {
     typedef typename F::T T;
-    if constexpr (r == 1) {
+    if constexpr (r == 3 || r == 5 || r == 7) {
+        static_assert(std::is_same<T, C64>::value, "odd radices are FP64 only");
+        dft_odd<r, INV>(x);
+    } else if constexpr (r == 1) {
         return;
     } else if constexpr (r == 2) {
         dft2<F, INV>(x[0], x[1]);
@@ -153,4 +219,64 @@ LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t,
         stockham_pass<F, M, G, R, R2, R1, false, false, INV>(x, lds, t, tw);
         stockham_pass<F, M, G, R, R3, R1 * R2, false, true, INV>(x, lds, t, tw);
     }
+}
+
+
+// ---- all-LDS mixed-radix Stockham transform (any radix list, any thread count) -----------
+// Used where the register-blocked form does not apply (lengths with an odd factor).  The data
+// sits in LDS in natural order (padded); every pass reads its butterflies' inputs, barriers,
+// and writes the outputs in place.  A thread runs ceil((H/r)/G) butterflies per pass.
+template <class F, int H, int G, int r, int L, bool INV>
+LL_DEV void mixed_pass(typename F::T *lds, int t, gptr<typename F::W> tw) {
+    typedef typename F::T T;
+    constexpr int NBF = H / r;
+    constexpr int NB = (NBF + G - 1) / G;
+    constexpr int TSTEP = H / (r * L);
+    static_assert(H % (r * L) == 0, "radix schedule does not divide the length");
+    T y[NB][r];
+    t = ll_opaque(t);
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int i = t + G * b;
+        if (i < NBF) {
+            const int k = i % L;
+#pragma unroll
+            for (int s = 0; s < r; ++s) y[b][s] = lds[lds_pad(i + s * NBF)];
+            if constexpr (L > 1) {
+#pragma unroll
+                for (int s = 1; s < r; ++s)
+                    y[b][s] = F::template twiddle<INV>(y[b][s], ld_table(tw, (u32)(s * k * TSTEP)));
+            }
+            dft_r<F, r, INV>(y[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int i = t + G * b;
+        if (i < NBF) {
+            const int k = i % L;
+#pragma unroll
+            for (int q = 0; q < r; ++q) lds[lds_pad((i - k) * r + k + q * L)] = y[b][q];
+        }
+    }
+}
+
+// x (column layout: thread t holds elements t + G s, s < R) -> transform -> x (same layout)
+template <class F, int H, int G, int R, bool INV, int R1, int R2, int R3, int R4>
+LL_DEV void mixed_transform(typename F::T (&x)[R], typename F::T *lds, int t, gptr<typename F::W> tw) {
+    static_assert(G * R == H, "column layout must cover the length");
+    static_assert(R1 * (R2 ? R2 : 1) * (R3 ? R3 : 1) * (R4 ? R4 : 1) == H, "radices must multiply to H");
+    const int tt = ll_opaque(t);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < R; ++s) lds[lds_pad(tt + G * s)] = x[s];
+    mixed_pass<F, H, G, R1, 1, INV>(lds, t, tw);
+    if constexpr (R2) mixed_pass<F, H, G, R2, R1, INV>(lds, t, tw);
+    if constexpr (R3) mixed_pass<F, H, G, R3, R1 * R2, INV>(lds, t, tw);
+    if constexpr (R4) mixed_pass<F, H, G, R4, R1 * R2 * R3, INV>(lds, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < R; ++s) x[s] = lds[lds_pad(tt + G * s)];
 }

@@ -37,8 +37,28 @@ struct Sched {
     static_assert(K >= 6 && K <= 11, "transform length out of range");
     static constexpr int R1 = 8;
     static constexpr int R2 = K == 6 ? 8 : K == 7 ? 2 : K == 8 ? 4 : K == 9 ? 8 : K == 10 ? 2 : 4;
-    static constexpr int R3 = K == 6 ? 0 : K <= 9 ? 8 : 8;
+    static constexpr int R3 = K == 6 ? 0 : 8;
     static constexpr int R4 = K <= 9 ? 0 : 8;
+};
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+constexpr bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Column (H-point) transform schedule.  Powers of two use the register-blocked Stockham
+// passes; lengths with an odd factor (3, 5, 7, 15) use the all-LDS mixed-radix passes.
+template <int H>
+struct ColSched {
+    static constexpr bool MIXED = !is_pow2(H);
+    static constexpr int G = H / kR;
+    static constexpr int R1 = MIXED ? (H % 15 == 0 ? 3 : H % 7 == 0 ? 7 : H % 5 == 0 ? 5 : 3) : Sched<ilog2c(H)>::R1;
+    static constexpr int REST = H / R1;                 // after the first odd pass (MIXED)
+    static constexpr int R2 = MIXED ? (H % 15 == 0 ? 5 : (REST % 16 == 0 && REST > 128 ? 16 : (REST % 8 == 0 ? 8 : 4)))
+                                    : Sched<ilog2c(H)>::R2;
+    static constexpr int REST2 = REST / R2;
+    static constexpr int R3 = MIXED ? (REST2 >= 16 && REST2 % 16 == 0 && REST2 > 64 ? 16 : REST2 >= 8 ? 8 : REST2)
+                                    : Sched<ilog2c(H)>::R3;
+    static constexpr int REST3 = R3 ? REST2 / R3 : 1;
+    static constexpr int R4 = MIXED ? (REST3 > 1 ? REST3 : 0) : Sched<ilog2c(H)>::R4;
 };
 
 template <bool IS_GL>
@@ -64,8 +84,10 @@ struct LargeArgs {
     const void *tmLi, *tmHi; // GL inverse of the above
     const void *wfL, *wfH;   // forward weight of exponent e: wfL[e & (Sw-1)] * wfH[e >> sw]
     const void *wiL, *wiH;   // inverse weight (with the transform's 1/M or 1/N)
-    u32 p, log_n, log_s, log_sw;
+    u32 p, log_n, log_s, log_sw;     // log_n: log2 N when N is a power of two, else 0
     u32 err_slot;
+    u64 n;                            // digit count N
+    double inv_n;                     // 1.0 / N
 };
 
 }  // namespace llh
@@ -89,10 +111,38 @@ LL_DEV double wt2d(const void *lo, const void *hi, u32 e, u32 log_s) {
     return ld_table(as_global<double>(lo), e & ((1u << log_s) - 1)) * ld_table(as_global<double>(hi), e >> log_s);
 }
 
-// c_d = ceil(d p / N) for power-of-two N
-LL_DEV u64 cbit(u64 d, u32 p, u32 log_n) { return (d * p + ((1ull << log_n) - 1)) >> log_n; }
-LL_DEV int bits_of(u64 d, u32 p, u32 log_n) { return (int)(cbit(d + 1, p, log_n) - cbit(d, p, log_n)); }
-LL_DEV u32 wexp(u64 d, u32 p, u32 log_n) { return (0u - (u32)(d * p)) & ((1u << log_n) - 1); }
+// Digit layout for any N: c_d = ceil(d p / N), b_d = c_{d+1} - c_d, weight exponent
+// e_d = c_d N - d p in [0, N).  x = d p < 2^53, so one double-precision quotient estimate
+// is within one of floor(x / N) and a single correction makes it exact.
+struct NDiv {
+    u64 n;
+    double inv;
+    u32 log_n;   // nonzero when N is a power of two
+};
+LL_DEV void divmod_n(u64 x, const NDiv &D, u64 &q, u64 &r) {
+    if (D.log_n) {
+        q = x >> D.log_n;
+        r = x & (D.n - 1);
+        return;
+    }
+    q = (u64)((double)x * D.inv);
+    i64 rr = (i64)(x - q * D.n);
+    if (rr < 0) { --q; rr += (i64)D.n; }
+    else if (rr >= (i64)D.n) { ++q; rr -= (i64)D.n; }
+    r = (u64)rr;
+}
+LL_DEV u64 cbit(u64 d, u32 p, const NDiv &D) {
+    u64 q, r;
+    divmod_n(d * p, D, q, r);
+    return q + (r != 0);
+}
+LL_DEV int bits_of(u64 d, u32 p, const NDiv &D) { return (int)(cbit(d + 1, p, D) - cbit(d, p, D)); }
+LL_DEV u32 wexp(u64 d, u32 p, const NDiv &D) {
+    u64 q, r;
+    divmod_n(d * p, D, q, r);
+    return r ? (u32)(D.n - r) : 0u;
+}
+LL_HD NDiv ndiv_of(const LargeArgs &A) { return NDiv{A.n, A.inv_n, A.log_n}; }
 
 // balanced split of v at b bits: v = x + c 2^b, x in [-2^(b-1), 2^(b-1))
 LL_DEV double split_d(double v, int b, double &c) {
@@ -105,14 +155,14 @@ LL_DEV i64 split_i(i64 v, int b, i64 &c) {
 }
 
 // ---- K1 -----------------------------------------------------------------------------------
-template <bool IS_GL, int LOG_W, int LOG_H>
-__global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_fwd(LargeArgs A) {
+template <bool IS_GL, int LOG_W, int H>
+__global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
     typedef typename E::CT CT;
-    typedef Sched<LOG_H> S;
-    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, GH = H / kR, NT = kCW * GH;
+    typedef ColSched<H> S;
+    constexpr int W = 1 << LOG_W, GH = H / kR, NT = kCW * GH;
     constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
     constexpr int NBLK = W / kCW;
     __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
@@ -121,7 +171,8 @@ __global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_fwd(LargeAr
     T *lds = lds_all + c * REG;
     T *elem = reinterpret_cast<T *>(A.elem);
     const CT *carry = reinterpret_cast<const CT *>(A.carry);
-    const u32 p = A.p, log_n = A.log_n;
+    const u32 p = A.p;
+    const NDiv log_n = ndiv_of(A);
 
     T x[kR];
 #pragma unroll
@@ -177,7 +228,10 @@ __global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_fwd(LargeAr
             }
         }
     }
-    transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, as_global<T>(A.twH));
+    if constexpr (S::MIXED)
+        mixed_transform<F, H, GH, kR, false, S::R1, S::R2, S::R3, S::R4>(x, lds, t, as_global<T>(A.twH));
+    else
+        transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, as_global<T>(A.twH));
     // ---- twiddle w_M^{w k_h}, k_h = t + GH q, and store
     {
         const int tt = ll_opaque(t);
@@ -193,13 +247,13 @@ __global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_fwd(LargeAr
 }
 
 // ---- K2 -----------------------------------------------------------------------------------
-template <bool IS_GL, int LOG_W, int LOG_H>
+template <bool IS_GL, int LOG_W, int H>
 __global__ void __launch_bounds__(2 * ((1 << LOG_W) / kR)) lg_rows(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
     typedef Sched<LOG_W> S;
-    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, GW = W / kR;
+    constexpr int W = 1 << LOG_W, GW = W / kR;
     constexpr int REG = W + W / 16 + 4;
     __shared__ __attribute__((aligned(16))) T lds_all[2 * REG];
     const int tid = threadIdx.x, g = tid / GW, t = tid % GW;
@@ -244,14 +298,14 @@ __global__ void __launch_bounds__(2 * ((1 << LOG_W) / kR)) lg_rows(LargeArgs A) 
 }
 
 // ---- K3 -----------------------------------------------------------------------------------
-template <bool IS_GL, int LOG_W, int LOG_H>
-__global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_inv(LargeArgs A) {
+template <bool IS_GL, int LOG_W, int H>
+__global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
     typedef typename E::CT CT;
-    typedef Sched<LOG_H> S;
-    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, GH = H / kR, NT = kCW * GH;
+    typedef ColSched<H> S;
+    constexpr int W = 1 << LOG_W, GH = H / kR, NT = kCW * GH;
     constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
     __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
     __shared__ unsigned long long s_err;
@@ -260,7 +314,8 @@ __global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_inv(LargeAr
     T *lds = lds_all + c * REG;
     T *elem = reinterpret_cast<T *>(A.elem);
     CT *carry = reinterpret_cast<CT *>(A.carry);
-    const u32 p = A.p, log_n = A.log_n;
+    const u32 p = A.p;
+    const NDiv log_n = ndiv_of(A);
     if (tid == 0) s_err = 0;
 
     T x[kR];
@@ -275,8 +330,10 @@ __global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_inv(LargeAr
             else x[s] = cmulc(x[s], tw2<C64>(A.tmL, A.tmH, e, A.log_s));
         }
     }
-    transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, true>(
-        x, lds, t, as_global<T>(IS_GL ? A.twHi : A.twH));
+    if constexpr (S::MIXED)
+        mixed_transform<F, H, GH, kR, true, S::R1, S::R2, S::R3, S::R4>(x, lds, t, as_global<T>(A.twH));
+    else
+        transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, as_global<T>(IS_GL ? A.twHi : A.twH));
     // ---- unweight and round (rows h = t + GH q)
     double eloc = 0.0;
     {
@@ -339,16 +396,17 @@ __global__ void __launch_bounds__(kCW * ((1 << LOG_H) / kR)) lg_cols_inv(LargeAr
 }
 
 // ---- carry-in only (after the last squaring, before normalisation) ----------------------------
-template <bool IS_GL, int LOG_W, int LOG_H>
+template <bool IS_GL, int LOG_W, int H>
 __global__ void __launch_bounds__(256) lg_apply_carry(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::T T;
     typedef typename E::CT CT;
-    constexpr int W = 1 << LOG_W, H = 1 << LOG_H, NBLK = W / kCW;
+    constexpr int W = 1 << LOG_W, NBLK = W / kCW;
     const int blk = blockIdx.x;
     T *elem = reinterpret_cast<T *>(A.elem);
     const CT *carry = reinterpret_cast<const CT *>(A.carry);
-    const u32 p = A.p, log_n = A.log_n;
+    const u32 p = A.p;
+    const NDiv log_n = ndiv_of(A);
     for (int h = threadIdx.x; h < H; h += 256) {
         CT cin;
         if (blk > 0) cin = carry[(blk - 1) * H + h];
@@ -395,9 +453,9 @@ struct NormOut {
 };
 
 template <bool IS_GL>
-__global__ void __launch_bounds__(kNormThreads) lg_normalize(const void *elem_v, i64 *dig, u32 p, u32 log_n,
+__global__ void __launch_bounds__(kNormThreads) lg_normalize(const void *elem_v, i64 *dig, u32 p, NDiv log_n,
                                                            NormOut *out) {
-    const u64 N = 1ull << log_n;
+    const u64 N = log_n.n;
     const u64 K = (N + kNormThreads - 1) / kNormThreads;
     const int t = threadIdx.x;
     const u64 d0 = (u64)t * K, d1 = d0 + K < N ? d0 + K : N;
@@ -473,37 +531,69 @@ struct LargeLaunch {
     void (*rows)(LargeArgs);
     void (*inv)(LargeArgs);
     void (*apply)(LargeArgs);
-    int log_w, log_h;
+    int log_w, log_h;   // log_h holds H itself (not its log) since H may have an odd factor
 };
 
-template <bool IS_GL, int LW, int LH>
+template <bool IS_GL, int LW, int H>
 static LargeLaunch make_launch() {
-    return LargeLaunch{lg_cols_fwd<IS_GL, LW, LH>, lg_rows<IS_GL, LW, LH>, lg_cols_inv<IS_GL, LW, LH>,
-                       lg_apply_carry<IS_GL, LW, LH>, LW, LH};
+    return LargeLaunch{lg_cols_fwd<IS_GL, LW, H>, lg_rows<IS_GL, LW, H>, lg_cols_inv<IS_GL, LW, H>,
+                       lg_apply_carry<IS_GL, LW, H>, LW, H};
 }
 
-// log2 of the element count M -> (W, H) split and kernels
-bool large_launch_for(int transform, int log_m, LargeLaunch &L) {
+// element count M -> (W, H) split and kernels.  FP64: M = m 2^k with m in {1, 3, 5, 7, 15};
+// GL: powers of two (the field has no 3rd or 7th roots of unity, and no N-th root of 2 for
+// N = 3 * 2^k).
+bool large_launch_for(int transform, u64 M, LargeLaunch &L) {
     const bool gl = transform == kNTT;
-#define LL_CASE(LM, LW, LH)                                              \
-    case LM:                                                             \
-        L = gl ? make_launch<true, LW, LH>() : make_launch<false, LW, LH>(); \
+#define LL_P2(MV, LW, H)                                                    \
+    case MV:                                                                \
+        L = gl ? make_launch<true, LW, H>() : make_launch<false, LW, H>();  \
         return true;
-    switch (log_m) {
-        LL_CASE(15, 8, 7)
-        LL_CASE(16, 8, 8)
-        LL_CASE(17, 9, 8)
-        LL_CASE(18, 9, 9)
-        LL_CASE(19, 10, 9)
-        LL_CASE(20, 10, 10)
-        LL_CASE(21, 11, 10)
+#define LL_ODD(MV, LW, H)                                                   \
+    case MV:                                                                \
+        if (gl) return false;                                               \
+        L = make_launch<false, LW, H>();                                    \
+        return true;
+    switch (M) {
+        LL_P2(1ull << 15, 8, 128)
+        LL_P2(1ull << 16, 8, 256)
+        LL_P2(1ull << 17, 9, 256)
+        LL_P2(1ull << 18, 9, 512)
+        LL_P2(1ull << 19, 10, 512)
+        LL_P2(1ull << 20, 10, 1024)
+        LL_P2(1ull << 21, 11, 1024)
+        LL_ODD(3ull << 14, 8, 192)
+        LL_ODD(3ull << 15, 9, 192)
+        LL_ODD(3ull << 16, 10, 192)
+        LL_ODD(3ull << 17, 10, 384)
+        LL_ODD(3ull << 18, 10, 768)
+        LL_ODD(3ull << 19, 11, 768)
+        LL_ODD(5ull << 13, 8, 160)
+        LL_ODD(5ull << 14, 9, 160)
+        LL_ODD(5ull << 15, 10, 160)
+        LL_ODD(5ull << 16, 10, 320)
+        LL_ODD(5ull << 17, 10, 640)
+        LL_ODD(5ull << 18, 11, 640)
+        LL_ODD(7ull << 13, 8, 224)
+        LL_ODD(7ull << 14, 9, 224)
+        LL_ODD(7ull << 15, 10, 224)
+        LL_ODD(7ull << 16, 10, 448)
+        LL_ODD(7ull << 17, 10, 896)
+        LL_ODD(7ull << 18, 11, 896)
+        LL_ODD(15ull << 12, 8, 240)
+        LL_ODD(15ull << 13, 9, 240)
+        LL_ODD(15ull << 14, 10, 240)
+        LL_ODD(15ull << 15, 10, 480)
+        LL_ODD(15ull << 16, 10, 960)
+        LL_ODD(15ull << 17, 11, 960)
         default: return false;
     }
-#undef LL_CASE
+#undef LL_P2
+#undef LL_ODD
 }
 
 hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st) {
-    const int W = 1 << L.log_w, H = 1 << L.log_h;
+    const int W = 1 << L.log_w, H = L.log_h;
     const dim3 gcol(W / kCW), bcol(kCW * (H / kR));
     const dim3 grow(H / 2), brow(2 * (W / kR));
     hipLaunchKernelGGL(L.fwd, gcol, bcol, 0, st, A);
@@ -517,13 +607,14 @@ hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream
     return hipGetLastError();
 }
 
-hipError_t large_normalize(int transform, const void *elem, i64 *dig, u32 p, u32 log_n, void *out,
+hipError_t large_normalize(int transform, const void *elem, i64 *dig, const LargeArgs &A, void *out,
                            hipStream_t st) {
+    const NDiv D = ndiv_of(A);
     if (transform == kNTT)
-        hipLaunchKernelGGL(lg_normalize<true>, dim3(1), dim3(kNormThreads), 0, st, elem, dig, p, log_n,
+        hipLaunchKernelGGL(lg_normalize<true>, dim3(1), dim3(kNormThreads), 0, st, elem, dig, A.p, D,
                            (NormOut *)out);
     else
-        hipLaunchKernelGGL(lg_normalize<false>, dim3(1), dim3(kNormThreads), 0, st, elem, dig, p, log_n,
+        hipLaunchKernelGGL(lg_normalize<false>, dim3(1), dim3(kNormThreads), 0, st, elem, dig, A.p, D,
                            (NormOut *)out);
     return hipGetLastError();
 }

@@ -57,6 +57,8 @@ def lib():
         L.llh_lucas_lehmer_n.argtypes = [u64, u64, ctypes.c_int, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32),
                                          P(i32), P(dbl)]
         L.llh_last_roundoff_trace.argtypes = [P(dbl), ctypes.c_size_t]
+        L.llh_supported_lengths.argtypes = [ctypes.c_int, P(u64), ctypes.c_size_t]
+        L.llh_supported_lengths.restype = ctypes.c_size_t
         L.llh_last_roundoff_trace.restype = ctypes.c_size_t
         L.llh_lucas_lehmer.argtypes = [u64, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32), P(i32)]
         _lib = L
@@ -138,3 +140,10 @@ def roundoff_trace() -> np.ndarray:
     out = np.zeros(n, np.float64)
     lib().llh_last_roundoff_trace(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
     return out
+
+
+def supported_lengths(transform: int = FP64) -> list[int]:
+    n = lib().llh_supported_lengths(transform, None, 0)
+    out = (ctypes.c_uint64 * n)()
+    lib().llh_supported_lengths(transform, out, n)
+    return [int(v) for v in out]

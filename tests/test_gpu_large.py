@@ -78,3 +78,25 @@ def test_default_length_routes_large_exponents():
         for i, p in enumerate(ps):
             assert out["status"][i] == llhip.ST_OK
             assert int(out["res64"][i]) == O.ll_gmp(p, 50)[1]
+
+
+# lengths with an odd factor (FP64 only): 3, 5, 7, 15 times a power of two, at 14-18 bits/digit
+ODD = [(3 << 15, 1572869), (5 << 14, 1310719), (7 << 14, 1835003), (15 << 13, 2088967),
+       (3 << 18, 12582917), (5 << 17, 10485767), (7 << 17, 16777213), (15 << 16, 16777213),
+       (15 << 17, 33554393), (7 << 18, 33554393)]
+
+
+@pytest.mark.parametrize("n,p", ODD, ids=[f"N{n}-p{p}" for n, p in ODD])
+def test_odd_factor_lengths_match_gmp(n, p):
+    k = 40 if n > 500000 else 150
+    r = llhip.lucas_lehmer_n(p, k, llhip.FP64, n, no_rollback=True)
+    assert r["status"] == llhip.ST_OK and r["n_used"] == n, r
+    assert r["res64"] == O.ll_gmp(p, k)[1]
+    assert r["maxerr"] < 0.4
+
+
+def test_conservative_length_rule_matches_reference():
+    # lucaslehmer.cpp:244-254 at 18 bits per word, through the C ABI
+    L = llhip.lib()
+    for p in (1048573, 16777213, 3021377, 20000003, 33554393):
+        assert L.llh_fft_length(p, 18.0) == O.fft_length(p, 18.0)
