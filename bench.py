@@ -1,21 +1,26 @@
 #!/usr/bin/env python3
-"""Headline benchmark: Lucas-Lehmer iterations/s of the batched GPU engine.
+"""Headline benchmark: Lucas-Lehmer iterations/s of the MI355X engines.
 
-Workload (one "step"): the full Lucas-Lehmer test of every prime 2 <= p <= 100001
-(9592 exponents, sum(p-2) = 454,377,353 squarings) in ONE persistent launch of the
-batched FP64 IBDWT engine, every Res64 checked against the committed GMP fixture.
+Default workload (one "step"): the full Lucas-Lehmer test of every prime 2 <= p <= 100001
+(9592 exponents, sum(p-2) = 454,377,353 squarings) in ONE persistent launch of the batched
+FP64 IBDWT engine, every Res64 checked against the committed GMP fixture.
 
-Prints one JSON line (value = iterations/s of the whole step, inputs on the device: the
-only input is the exponent list; residues are created and kept on the GPU), with:
-  roofline      FP64 VALU bound of the batch kernel: algorithmic flops per launch
-                (sum over exponents of (p-2) * (5 N log2 N + 6.5 N)) / kernel time
-                (HIP events on the library's stream), against MI355X's 78.6 TFLOP/s FP64;
-                traffic from the committed rocprofv3 PMC summary when present.
-  cpu_baseline  the CPU oracle's GMP engine (a restatement of lucaslehmer.cpp:29-56, one
-                exponent per thread as PerfNumMultiCLL.py:189-193) on a bounded sample.
+Prints one JSON line: value = iterations/s of the whole step (the only input is the
+exponent list; residues are created and stay on the GPU), plus
+  roofline      the batch kernel against MI355X's FP64 vector peak (78.6 TFLOP/s):
+                algorithmic flops per launch = sum over exponents of (p-2)(5 N log2 N + 6.5 N)
+                divided by the kernel time (HIP events on the library's stream); traffic from
+                the committed rocprofv3 PMC summary when present.
+  cpu_baseline  the CPU oracle's GMP engine (restates lucaslehmer.cpp:29-56), one exponent per
+                thread (PerfNumMultiCLL.py:189-193), on a bounded sample of the same list.
+  extra         the other configurations, each measured briefly: the single large exponent at
+                p = 1,048,573 (N = 2^16), p = 16,777,213 on the FP64 engine at the conservative
+                length and at the learned length, the same on the exact NTT, the batched NTT
+                over the same 9592 exponents, and the plain one-exponent-per-launch form.
 
-The library call is synchronous (it waits on its own HIP stream), so the timed region is
-bracketed by completed device work on both sides.  One GPU only: --gpus > 1 exits.
+--workload selects one configuration alone (used for the rocprofv3 runs).  The library
+calls are synchronous (they wait on the library's own HIP stream), so every timed region
+is bracketed by completed device work.  One GPU only: --gpus > 1 exits.
 """
 from __future__ import annotations
 
@@ -31,13 +36,29 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+P24 = 16777213
+P20 = 1048573
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def load_fixture(name: str):
+def load_csv(name: str):
+    """{(p, iters): res64} from an interim-residue fixture."""
+    path = os.path.join(ROOT, "tests", "golden", name)
+    if not os.path.exists(path):
+        return None
+    out = {}
+    with open(path) as f:
+        next(f)
+        for line in f:
+            a = line.strip().split(",")
+            out[(int(a[0]), int(a[1]))] = int(a[2], 16)
+    return out
+
+
+def load_range(name: str):
     path = os.path.join(ROOT, "tests", "golden", name)
     if not os.path.exists(path):
         return None
@@ -61,10 +82,33 @@ def cpu_baseline(ps: list[int], every: int, threads: int) -> dict:
     t0 = time.perf_counter()
     O.ll_many(sample, engine=0, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": iters / dt, "unit": "LL iters/s", "cores": threads, "kind": "port",
+    return {"value": round(iters / dt, 1), "unit": "LL iters/s", "cores": threads, "kind": "port",
             "sample": f"GMP engine (restates lucaslehmer.cpp:29-56), full LL of every {every}th prime "
                       f"<= 100001 ({len(sample)} exponents, {iters} squarings, {dt:.1f} s wall), "
                       f"one exponent per thread, GMP {O.gmp_version()}"}
+
+
+def run_single(llhip, p: int, iters: int, transform: int, n: int = 0) -> dict:
+    t0 = time.perf_counter()
+    r = llhip.lucas_lehmer_n(p, iters, transform, n)
+    wall = time.perf_counter() - t0
+    r["wall"] = wall
+    r["iters"] = iters
+    r["iters_per_s"] = iters / wall
+    return r
+
+
+def learned_length(p: int) -> int:
+    """Length the trained round-off model picks for p (0 when no model is committed)."""
+    try:
+        from mersenneprime_python_cpp_gmp_benchmark_amd import nselect
+    except ImportError:
+        return 0
+    try:
+        return nselect.choose_length(p)
+    except Exception as e:                         # no model file yet
+        log(f"learned length unavailable: {e}")
+        return 0
 
 
 def main() -> int:
@@ -72,11 +116,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--transform", choices=["fp64", "ntt"], default="fp64")
-    ap.add_argument("--limit", type=int, default=100001, help="largest exponent of the workload")
+    ap.add_argument("--workload", choices=["c3", "c3-ntt", "c2", "p24-fp64", "p24-ntt"], default="c3")
+    ap.add_argument("--n", type=int, default=0, help="p24-fp64: transform length (0 = conservative rule)")
+    ap.add_argument("--iters", type=int, default=0, help="squarings for the single-exponent workloads")
     ap.add_argument("--cpu-every", type=int, default=200, help="CPU baseline sample: every k-th prime")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); the exponents of a batch are "
@@ -88,55 +134,129 @@ def main() -> int:
     add_dropin_path()
     import lucaslehmer as ll
 
-    tr = llhip.FP64 if a.transform == "fp64" else llhip.NTT
     dev = llhip.device_info()
-    ps = ll.primes(2, a.limit)
-    total_iters = sum(p - 2 for p in ps if p > 2)
-    log(f"device {dev['name']} ({dev['arch']}, {dev['cus']} CUs); {len(ps)} exponents, {total_iters} squarings")
+    log(f"device {dev['name']} ({dev['arch']}, {dev['cus']} CUs)")
+    interim = load_csv("big_gmp_interim.csv") or {}
 
+    # ---- single-exponent workloads (profiling targets) ----------------------------------
+    if a.workload in ("c2", "p24-fp64", "p24-ntt"):
+        if a.workload == "c2":
+            p, tr, n, k = P20, llhip.FP64, 1 << 16, a.iters or 10000
+        elif a.workload == "p24-fp64":
+            p, tr, n, k = P24, llhip.FP64, a.n, a.iters or 5000
+        else:
+            p, tr, n, k = P24, llhip.NTT, 1 << 20, a.iters or 5000
+        for _ in range(a.warmup):
+            run_single(llhip, p, min(k, 400), tr, n)
+        r = run_single(llhip, p, k, tr, n)
+        want = interim.get((p, k))
+        line = {"metric": "LL iters/s", "value": round(r["iters_per_s"], 1), "unit": "LL iters/s", "n_gpus": 1,
+                "steps": 1, "warmup": a.warmup, "ms_per_step": round(r["wall"] * 1e3, 3), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f64" if tr == llhip.FP64 else "u64",
+                "data": "synthetic (s_0 = 4)",
+                "config": {"workload": a.workload, "p": p, "n": r["n_used"], "iters": k},
+                "res64": f"{r['res64']:016x}", "res64_matches_gmp": (want == r["res64"]) if want is not None else None,
+                "maxerr": r["maxerr"], "status": r["status"]}
+        print(json.dumps(line), flush=True)
+        return 0
+
+    # ---- batched workloads ---------------------------------------------------------------
+    tr = llhip.NTT if a.workload == "c3-ntt" else llhip.FP64
+    ps = ll.primes(2, 100001)
+    total_iters = sum(p - 2 for p in ps if p > 2)
+    log(f"{len(ps)} exponents, {total_iters} squarings")
     for w in range(a.warmup):
         t0 = time.perf_counter()
         llhip.batch(ps, 0, tr)
         log(f"warmup {w}: {time.perf_counter() - t0:.2f} s")
-
     kernel_ms = 0.0
     launches = 0
     t0 = time.perf_counter()
-    for s in range(a.steps):
+    for _ in range(a.steps):
         out = llhip.batch(ps, 0, tr)
         ms, nl = llhip.last_launch_stats()
         kernel_ms += ms
         launches += nl
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    wall = time.perf_counter() - t0
     log(f"timed {a.steps} step(s): {wall:.3f} s, kernel {kernel_ms:.1f} ms over {launches} launch(es)")
 
-    fx = load_fixture("c3_gmp_p100001.csv")
-    diffs = None
+    fx = load_range("c3_gmp_p100001.csv")
+    fx_partial = False
+    if fx is None:           # fixture still being generated: check the rows that exist
+        path = os.path.join(ROOT, "tests", "golden", "c3_gmp_p100001.csv.partial")
+        if os.path.exists(path):
+            fx = {}
+            for line_ in open(path):
+                p_, v_, r_ = line_.strip().split(",")
+                fx[int(p_)] = (int(v_), int(r_, 16))
+            fx_partial = True
+    diffs, checked = None, 0
     if fx is not None:
         diffs = 0
         for p, v, r in zip(ps, out["verdict"], out["res64"]):
-            if p in fx and (fx[p][1] != int(r) or fx[p][0] != int(v != 0)):
-                diffs += 1
+            if p in fx:
+                checked += 1
+                if fx[p][1] != int(r) or fx[p][0] != int(v != 0):
+                    diffs += 1
     bad_status = int(sum(1 for s in out["status"] if s not in (llhip.ST_OK, llhip.ST_TRIVIAL)))
 
-    n_used = out["n_used"]
     flops = 0.0
-    for p, n in zip(ps, n_used):
+    for p, n in zip(ps, out["n_used"]):
         if p > 2 and n > 1:
             flops += (p - 2) * fp64_flops_per_iter(int(n))
     per_launch_s = kernel_ms / 1e3 / max(a.steps, 1)
     achieved = flops / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_batch_fp64_p100001.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_ll_batch_kernel_fp64_p100001.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "kernel": "ll_batch_kernel<FP64>" if tr == llhip.FP64 else "ll_batch_kernel<NTT>",
+                "kernel": "ll_batch_kernel<false> (FP64)" if tr == llhip.FP64 else "ll_batch_kernel<true> (NTT)",
                 "kernel_ms_per_launch": round(per_launch_s * 1e3, 3),
-                "flops_per_launch": flops if tr == llhip.FP64 else None}
+                "flops_per_launch": flops,
+                "note": "FP64 vector-ALU bound (no MFMA on this path); flops counted at the FP64 length "
+                        "actually used per exponent"}
+
+    extra = {}
+    if not a.no_extra and a.workload == "c3":
+        # C2: p = 1,048,573 at the conservative N = 2^16, 10,000 squarings
+        r = run_single(llhip, P20, 10000, llhip.FP64, 1 << 16)
+        extra["c2_p1048573_N65536_fp64"] = {
+            "iters_per_s": round(r["iters_per_s"], 1), "us_per_iter": round(r["wall"] / 10000 * 1e6, 2),
+            "maxerr": r["maxerr"], "res64": f"{r['res64']:016x}",
+            "res64_matches_gmp": interim.get((P20, 10000)) == r["res64"] if (P20, 10000) in interim else None,
+            "hbm_alg_GBs": round(16 * (1 << 16) / (r["wall"] / 10000) / 1e9, 1)}
+        # p ~ 2^24: plain (conservative length) vs learned length, FP64; and the exact NTT
+        k24 = 5000
+        plain = run_single(llhip, P24, k24, llhip.FP64, 0)
+        ent = {"plain_conservative": {"n": plain["n_used"], "iters_per_s": round(plain["iters_per_s"], 1),
+                                      "maxerr": plain["maxerr"], "res64": f"{plain['res64']:016x}"}}
+        nl = learned_length(P24)
+        if nl:
+            lr = run_single(llhip, P24, k24, llhip.FP64, nl)
+            ent["learned"] = {"n": lr["n_used"], "iters_per_s": round(lr["iters_per_s"], 1),
+                              "maxerr": lr["maxerr"], "res64": f"{lr['res64']:016x}",
+                              "res64_equal_plain": lr["res64"] == plain["res64"],
+                              "speedup_vs_plain": round(lr["iters_per_s"] / plain["iters_per_s"], 3)}
+        ntt = run_single(llhip, P24, k24, llhip.NTT, 1 << 20)
+        ent["ntt_N1048576"] = {"iters_per_s": round(ntt["iters_per_s"], 1), "res64": f"{ntt['res64']:016x}",
+                               "res64_equal_fp64": ntt["res64"] == plain["res64"],
+                               "res64_matches_gmp": interim.get((P24, k24)) == ntt["res64"]
+                               if (P24, k24) in interim else None}
+        extra["p16777213_5000_iters"] = ent
+        # plain GPU form of the batch: one exponent per launch, a sample across the range
+        sample = ps[len(ps) // 16::len(ps) // 8][:8]
+        t1 = time.perf_counter()
+        si = 0
+        for p in sample:
+            llhip.batch([p], 0, tr)
+            si += p - 2
+        dt1 = time.perf_counter() - t1
+        extra["plain_one_exponent_per_launch"] = {
+            "sample": sample, "iters_per_s": round(si / dt1, 1),
+            "batched_speedup": round((a.steps * total_iters / wall) / (si / dt1), 1)}
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -151,12 +271,12 @@ def main() -> int:
         "ms_per_step": round(wall / a.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if tr == llhip.FP64 else "u64 (GF(2^64-2^32+1))",
         "data": "synthetic (deterministic: the exponent list; s_0 = 4)",
-        "config": {"workload": f"batched full Lucas-Lehmer of all {len(ps)} primes p <= {a.limit} in one "
-                               f"persistent launch ({a.transform} IBDWT)",
-                   "exponents": len(ps), "squarings": total_iters, "transform": a.transform,
-                   "bits_per_word_cap": llhip.get_bpw()},
-        "res64_diffs_vs_gmp": diffs, "not_ok_status": bad_status,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "config": {"workload": f"batched full Lucas-Lehmer of all {len(ps)} primes p <= 100001 in one "
+                               f"persistent launch ({'fp64' if tr == llhip.FP64 else 'ntt'} IBDWT)",
+                   "exponents": len(ps), "squarings": total_iters, "bits_per_word_cap": llhip.get_bpw()},
+        "res64_diffs_vs_gmp": diffs, "res64_checked": checked, "gmp_fixture_partial": fx_partial,
+        "not_ok_status": bad_status,
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(line), flush=True)
     return 0
