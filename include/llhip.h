@@ -99,6 +99,19 @@ size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap);
  * maximum over the run (lucaslehmer.cpp:289-295). */
 size_t llh_last_roundoff_trace(double *out, size_t cap);
 
+/* ---- round-off model: MLP 3 -> 64 -> 64 -> 1 (FP32, tanh), trained on the GPU ----------------
+ * No reference counterpart (the reference fixes N with a bits-per-word cap, lucaslehmer.cpp:246).
+ * Inputs are rows of 3 floats (the caller's normalised log2 p, log2 N, bits per digit), targets
+ * the normalised log10 of the worst round-off.  Parameters: W1[64][3], b1[64], W2[64][64],
+ * b2[64], w3[64], b3 (llh_mlp_param_count floats). */
+int llh_mlp_param_count(void);
+/* gradient of the mean squared error over n samples (and the loss) at `params` */
+int llh_mlp_grad(const float *params, const float *X, const float *y, size_t n, float *grad, float *loss);
+/* full-batch Adam for `epochs` steps from `params` (updated in place); loss_hist[epochs] optional */
+int llh_mlp_train(float *params, const float *X, const float *y, size_t n, int epochs, float lr, float *loss_hist);
+/* predictions for n inputs */
+int llh_mlp_predict(const float *params, const float *X, size_t n, float *y_out);
+
 /* One exponent.  transform = LLH_NTT replaces lucas_lehmer_n lucaslehmer.cpp:29-56 (the exact
  * engine; Res64 of gmp_res64_after :364-365); LLH_FP64 replaces lucas_lehmer_fft_raw
  * lucaslehmer.cpp:256-329 (maxerr of fft_info :337-342 / fft_res64_after :361-363).

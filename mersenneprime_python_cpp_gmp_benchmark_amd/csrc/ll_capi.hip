@@ -20,6 +20,16 @@
 
 namespace llh {
 
+int mlp_n_params();
+struct MlpBuffers {
+    float *P, *X, *T, *Y, *H1, *H2, *DZ1, *DZ2, *DY, *G, *Mm, *Vv, *loss;
+};
+hipError_t mlp_alloc(MlpBuffers &B, size_t n);
+void mlp_free(MlpBuffers &B);
+hipError_t mlp_forward(const MlpBuffers &B, int n, bool with_backward, hipStream_t st);
+hipError_t mlp_gradient(const MlpBuffers &B, int n, hipStream_t st);
+hipError_t mlp_step(const MlpBuffers &B, float lr, int t, hipStream_t st);
+
 hipError_t launch_batch(int transform, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
                         BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream);
 int batch_blocks_per_cu(int transform);
@@ -44,7 +54,7 @@ struct LargeLaunch {
     void (*rows)(LargeArgs);
     void (*inv)(LargeArgs);
     void (*apply)(LargeArgs);
-    int log_w, log_h;
+    int w_len, h_len;
 };
 bool large_launch_for(int transform, u64 M, LargeLaunch &L);
 hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
@@ -355,7 +365,7 @@ static int get_large_tables(int transform, u64 N, LargeTables &out) {
         set_error("no large-engine kernels for this length");
         return LLH_ERR_UNSUPPORTED;
     }
-    const u64 W = (u64)1 << L.log_w, H = (u64)L.log_h;
+    const u64 W = (u64)L.w_len, H = (u64)L.h_len;
     LargeTables T;
     T.log_s = (u32)((ceil_log2(M) + 1) / 2);
     T.log_sw = (u32)((ceil_log2(N) + 1) / 2);
@@ -437,7 +447,7 @@ static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
     LargeTables T;
     int rc = get_large_tables(transform, N, T);
     if (rc) return rc;
-    const u64 H = (u64)L.log_h, W = (u64)1 << L.log_w;
+    const u64 H = (u64)L.h_len, W = (u64)L.w_len;
     const int log_n = (N & (N - 1)) ? 0 : ceil_log2(N);
     const size_t esz = transform == kFP64 ? 16 : 8;
     const u64 nblk = W / 4;
@@ -800,6 +810,80 @@ size_t llh_last_roundoff_trace(double *out, size_t cap) {
     if (out)
         for (size_t i = 0; i < n && i < cap; ++i) out[i] = g_err_trace[i];
     return n;
+}
+
+int llh_mlp_param_count(void) { return mlp_n_params(); }
+
+// shared set-up of the MLP entry points: buffers sized for n samples, parameters and data
+static int mlp_setup(MlpBuffers &B, const float *params, const float *X, const float *y, size_t n) {
+    std::memset(&B, 0, sizeof(B));
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!params || !X || n == 0 || n > (1u << 24)) {
+        set_error("mlp: bad arguments");
+        return LLH_ERR_ARG;
+    }
+    HIPCHK(mlp_alloc(B, n));
+    hipStream_t st = g_dev.stream;
+    HIPCHK(hipMemcpyAsync(B.P, params, mlp_n_params() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.X, X, 3 * n * 4, hipMemcpyHostToDevice, st));
+    if (y) HIPCHK(hipMemcpyAsync(B.T, y, n * 4, hipMemcpyHostToDevice, st));
+    return LLH_OK;
+}
+
+int llh_mlp_grad(const float *params, const float *X, const float *y, size_t n, float *grad, float *loss) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    MlpBuffers B;
+    int rc = mlp_setup(B, params, X, y, n);
+    if (rc) { mlp_free(B); return rc; }
+    if (!y || !grad) { mlp_free(B); set_error("mlp: NULL output"); return LLH_ERR_ARG; }
+    hipStream_t st = g_dev.stream;
+    float l = 0;
+    HIPCHK(mlp_forward(B, (int)n, true, st));
+    HIPCHK(mlp_gradient(B, (int)n, st));
+    HIPCHK(hipMemcpyAsync(grad, B.G, mlp_n_params() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&l, B.loss, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (loss) *loss = l;
+    mlp_free(B);
+    return LLH_OK;
+}
+
+int llh_mlp_train(float *params, const float *X, const float *y, size_t n, int epochs, float lr, float *loss_hist) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    MlpBuffers B;
+    int rc = mlp_setup(B, params, X, y, n);
+    if (rc) { mlp_free(B); return rc; }
+    if (!y || epochs < 1) { mlp_free(B); set_error("mlp: bad arguments"); return LLH_ERR_ARG; }
+    hipStream_t st = g_dev.stream;
+    HIPCHK(hipMemsetAsync(B.Mm, 0, mlp_n_params() * 4, st));
+    HIPCHK(hipMemsetAsync(B.Vv, 0, mlp_n_params() * 4, st));
+    std::vector<float> hist(loss_hist ? epochs : 0);
+    for (int t = 1; t <= epochs; ++t) {
+        HIPCHK(mlp_forward(B, (int)n, true, st));
+        HIPCHK(mlp_gradient(B, (int)n, st));
+        if (loss_hist) HIPCHK(hipMemcpyAsync(&hist[t - 1], B.loss, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(mlp_step(B, lr, t, st));
+    }
+    HIPCHK(hipMemcpyAsync(params, B.P, mlp_n_params() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (loss_hist) std::memcpy(loss_hist, hist.data(), epochs * 4);
+    mlp_free(B);
+    return LLH_OK;
+}
+
+int llh_mlp_predict(const float *params, const float *X, size_t n, float *y_out) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    MlpBuffers B;
+    int rc = mlp_setup(B, params, X, nullptr, n);
+    if (rc) { mlp_free(B); return rc; }
+    if (!y_out) { mlp_free(B); set_error("mlp: NULL output"); return LLH_ERR_ARG; }
+    hipStream_t st = g_dev.stream;
+    HIPCHK(mlp_forward(B, (int)n, false, st));
+    HIPCHK(hipMemcpyAsync(y_out, B.Y, n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    mlp_free(B);
+    return LLH_OK;
 }
 
 int llh_last_launch_stats(double *kernel_ms, uint32_t *launches) {

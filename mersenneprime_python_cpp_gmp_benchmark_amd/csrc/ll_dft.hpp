@@ -280,3 +280,19 @@ LL_DEV void mixed_transform(typename F::T (&x)[R], typename F::T *lds, int t, gp
 #pragma unroll
     for (int s = 0; s < R; ++s) x[s] = lds[lds_pad(tt + G * s)];
 }
+
+
+// ---- register first/last passes around odd middle passes -------------------------------
+// For a length W = 8 * m * 8 (m = product of M1 M2 M3, any of 2..16 including 3, 5, 7):
+// the first radix-8 pass reads the column layout from registers, the middle passes run
+// through LDS (mixed_pass), the last radix-8 pass leaves the column layout in registers.
+template <class F, int W, int G, int R, bool INV, int M1, int M2, int M3>
+LL_DEV void hybrid_transform(typename F::T (&x)[R], typename F::T *lds, int t, gptr<typename F::W> tw) {
+    static_assert(R == 8 && G * R == W, "hybrid transform runs 8 elements per thread");
+    static_assert(8 * M1 * (M2 ? M2 : 1) * (M3 ? M3 : 1) * 8 == W, "radices must multiply to W");
+    stockham_pass<F, W, G, R, 8, 1, true, false, INV>(x, lds, t, tw);
+    mixed_pass<F, W, G, M1, 8, INV>(lds, t, tw);
+    if constexpr (M2 != 0) mixed_pass<F, W, G, M2, 8 * M1, INV>(lds, t, tw);
+    if constexpr (M3 != 0) mixed_pass<F, W, G, M3, 8 * M1 * M2, INV>(lds, t, tw);
+    stockham_pass<F, W, G, R, 8, W / 8, false, true, INV>(x, lds, t, tw);
+}

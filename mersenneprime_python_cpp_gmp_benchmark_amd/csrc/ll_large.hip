@@ -61,6 +61,20 @@ struct ColSched {
     static constexpr int R4 = MIXED ? (REST3 > 1 ? REST3 : 0) : Sched<ilog2c(H)>::R4;
 };
 
+// Row (W-point) transform schedule: powers of two use the register Stockham passes; W = 64 m
+// with m in {3, 5, 6, 7, 10, 12, 14, 15, 20, 24, 28, 30} uses hybrid_transform with the middle
+// factor m split into radices (odd ones first).
+template <int W>
+struct RowSched {
+    static constexpr bool HYBRID = !is_pow2(W);
+    static constexpr int MID = HYBRID ? W / 64 : 1;
+    static constexpr int odd = MID % 15 == 0 ? 15 : MID % 7 == 0 ? 7 : MID % 5 == 0 ? 5 : MID % 3 == 0 ? 3 : 1;
+    static constexpr int pw = MID / odd;                       // power-of-two part
+    static constexpr int M1 = odd == 15 ? 3 : odd;
+    static constexpr int M2 = odd == 15 ? 5 : (pw > 1 ? pw : 0);
+    static constexpr int M3 = odd == 15 ? (pw > 1 ? pw : 0) : 0;
+};
+
 template <bool IS_GL>
 struct ElemOf {
     typedef typename std::conditional<IS_GL, FGL, FC64>::type F;
@@ -155,14 +169,14 @@ LL_DEV i64 split_i(i64 v, int b, i64 &c) {
 }
 
 // ---- K1 -----------------------------------------------------------------------------------
-template <bool IS_GL, int LOG_W, int H>
+template <bool IS_GL, int W, int H>
 __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
     typedef typename E::CT CT;
     typedef ColSched<H> S;
-    constexpr int W = 1 << LOG_W, GH = H / kR, NT = kCW * GH;
+    constexpr int GH = H / kR, NT = kCW * GH;
     constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
     constexpr int NBLK = W / kCW;
     __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
@@ -246,14 +260,25 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     }
 }
 
+template <class F, int W, int GW, bool INV>
+LL_DEV void row_transform(typename F::T (&x)[kR], typename F::T *lds, int t, gptr<typename F::W> tw) {
+    typedef RowSched<W> RS;
+    if constexpr (RS::HYBRID) {
+        hybrid_transform<F, W, GW, kR, INV, RS::M1, RS::M2, RS::M3>(x, lds, t, tw);
+    } else {
+        typedef Sched<ilog2c(W)> S;
+        transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, INV>(x, lds, t, tw);
+    }
+}
+
 // ---- K2 -----------------------------------------------------------------------------------
-template <bool IS_GL, int LOG_W, int H>
-__global__ void __launch_bounds__(2 * ((1 << LOG_W) / kR)) lg_rows(LargeArgs A) {
+template <bool IS_GL, int W, int H>
+__global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
-    typedef Sched<LOG_W> S;
-    constexpr int W = 1 << LOG_W, GW = W / kR;
+    typedef RowSched<W> RS;
+    constexpr int GW = W / kR;
     constexpr int REG = W + W / 16 + 4;
     __shared__ __attribute__((aligned(16))) T lds_all[2 * REG];
     const int tid = threadIdx.x, g = tid / GW, t = tid % GW;
@@ -265,11 +290,11 @@ __global__ void __launch_bounds__(2 * ((1 << LOG_W) / kR)) lg_rows(LargeArgs A) 
     T x[kR];
 #pragma unroll
     for (int s = 0; s < kR; ++s) x[s] = elem[t + GW * s];
-    transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, as_global<T>(A.twW));
+    row_transform<F, W, GW, false>(x, lds, t, as_global<T>(A.twW));
     if constexpr (IS_GL) {
 #pragma unroll
         for (int q = 0; q < kR; ++q) x[q] = gl_sqr(x[q]);
-        transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, as_global<T>(A.twWi));
+        row_transform<F, W, GW, true>(x, lds, t, as_global<T>(A.twWi));
     } else {
         const int tt = ll_opaque(t);
         __syncthreads();
@@ -282,7 +307,7 @@ __global__ void __launch_bounds__(2 * ((1 << LOG_W) / kR)) lg_rows(LargeArgs A) 
 #pragma unroll
         for (int q = 0; q < kR; ++q) {
             const int kw = tt + GW * q;
-            const int kwp = row == 0 ? ((W - kw) & (W - 1)) : (W - 1 - kw);
+            const int kwp = row == 0 ? (kw == 0 ? 0 : W - kw) : (W - 1 - kw);
             const C64 zm = plds[lds_pad(kwp)];
             const C64 D = csub(x[q], cconj(zm));
             const u32 k = (u32)row + (u32)H * (u32)kw;
@@ -291,21 +316,21 @@ __global__ void __launch_bounds__(2 * ((1 << LOG_W) / kR)) lg_rows(LargeArgs A) 
             const C64 Z2 = csqr(x[q]);
             x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
         }
-        transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, as_global<T>(A.twW));
+        row_transform<F, W, GW, true>(x, lds, t, as_global<T>(A.twW));
     }
 #pragma unroll
     for (int s = 0; s < kR; ++s) elem[t + GW * s] = x[s];
 }
 
 // ---- K3 -----------------------------------------------------------------------------------
-template <bool IS_GL, int LOG_W, int H>
+template <bool IS_GL, int W, int H>
 __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
     typedef typename E::CT CT;
     typedef ColSched<H> S;
-    constexpr int W = 1 << LOG_W, GH = H / kR, NT = kCW * GH;
+    constexpr int GH = H / kR, NT = kCW * GH;
     constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
     __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
     __shared__ unsigned long long s_err;
@@ -396,12 +421,12 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
 }
 
 // ---- carry-in only (after the last squaring, before normalisation) ----------------------------
-template <bool IS_GL, int LOG_W, int H>
+template <bool IS_GL, int W, int H>
 __global__ void __launch_bounds__(256) lg_apply_carry(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::T T;
     typedef typename E::CT CT;
-    constexpr int W = 1 << LOG_W, NBLK = W / kCW;
+    constexpr int NBLK = W / kCW;
     const int blk = blockIdx.x;
     T *elem = reinterpret_cast<T *>(A.elem);
     const CT *carry = reinterpret_cast<const CT *>(A.carry);
@@ -531,61 +556,60 @@ struct LargeLaunch {
     void (*rows)(LargeArgs);
     void (*inv)(LargeArgs);
     void (*apply)(LargeArgs);
-    int log_w, log_h;   // log_h holds H itself (not its log) since H may have an odd factor
+    int w_len, h_len;   // W and H (either may carry the odd factor)
 };
 
-template <bool IS_GL, int LW, int H>
+template <bool IS_GL, int W, int H>
 static LargeLaunch make_launch() {
-    return LargeLaunch{lg_cols_fwd<IS_GL, LW, H>, lg_rows<IS_GL, LW, H>, lg_cols_inv<IS_GL, LW, H>,
-                       lg_apply_carry<IS_GL, LW, H>, LW, H};
+    return LargeLaunch{lg_cols_fwd<IS_GL, W, H>, lg_rows<IS_GL, W, H>, lg_cols_inv<IS_GL, W, H>,
+                       lg_apply_carry<IS_GL, W, H>, W, H};
 }
 
-// element count M -> (W, H) split and kernels.  FP64: M = m 2^k with m in {1, 3, 5, 7, 15};
-// GL: powers of two (the field has no 3rd or 7th roots of unity, and no N-th root of 2 for
-// N = 3 * 2^k).
+// element count M -> (W, H) split and kernels.  FP64: M = m 2^k with m in {1, 3, 5, 7, 15},
+// the odd factor in the row length W; GL: powers of two (the field has no 3rd or 7th roots of
+// unity, and no N-th root of 2 for N = 3 * 2^k).
 bool large_launch_for(int transform, u64 M, LargeLaunch &L) {
     const bool gl = transform == kNTT;
-#define LL_P2(MV, LW, H)                                                    \
-    case MV:                                                                \
-        L = gl ? make_launch<true, LW, H>() : make_launch<false, LW, H>();  \
+#define LL_P2(MV, W, H)                                                   \
+    case MV:                                                              \
+        L = gl ? make_launch<true, W, H>() : make_launch<false, W, H>();  \
         return true;
-#define LL_ODD(MV, LW, H)                                                   \
-    case MV:                                                                \
-        if (gl) return false;                                               \
-        L = make_launch<false, LW, H>();                                    \
+#define LL_ODD(MV, W, H)                                                  \
+    case MV:                                                              \
+        if (gl) return false;                                             \
+        L = make_launch<false, W, H>();                                   \
         return true;
     switch (M) {
-        LL_P2(1ull << 15, 8, 128)
-        LL_P2(1ull << 16, 8, 256)
-        LL_P2(1ull << 17, 9, 256)
-        LL_P2(1ull << 18, 9, 512)
-        LL_P2(1ull << 19, 10, 512)
-        LL_P2(1ull << 20, 10, 1024)
-        LL_P2(1ull << 21, 11, 1024)
-        LL_ODD(3ull << 14, 8, 192)
-        LL_ODD(3ull << 15, 9, 192)
-        LL_ODD(3ull << 16, 10, 192)
-        LL_ODD(3ull << 17, 10, 384)
-        LL_ODD(3ull << 18, 10, 768)
-        LL_ODD(3ull << 19, 11, 768)
-        LL_ODD(5ull << 13, 8, 160)
-        LL_ODD(5ull << 14, 9, 160)
-        LL_ODD(5ull << 15, 10, 160)
-        LL_ODD(5ull << 16, 10, 320)
-        LL_ODD(5ull << 17, 10, 640)
-        LL_ODD(5ull << 18, 11, 640)
-        LL_ODD(7ull << 13, 8, 224)
-        LL_ODD(7ull << 14, 9, 224)
-        LL_ODD(7ull << 15, 10, 224)
-        LL_ODD(7ull << 16, 10, 448)
-        LL_ODD(7ull << 17, 10, 896)
-        LL_ODD(7ull << 18, 11, 896)
-        LL_ODD(15ull << 12, 8, 240)
-        LL_ODD(15ull << 13, 9, 240)
-        LL_ODD(15ull << 14, 10, 240)
-        LL_ODD(15ull << 15, 10, 480)
-        LL_ODD(15ull << 16, 10, 960)
-        LL_ODD(15ull << 17, 11, 960)
+        LL_P2(1ull << 15, 256, 128)
+        LL_P2(1ull << 16, 256, 256)
+        LL_P2(1ull << 17, 512, 256)
+        LL_P2(1ull << 18, 512, 512)
+        LL_P2(1ull << 19, 1024, 512)
+        LL_P2(1ull << 20, 1024, 1024)
+        LL_P2(1ull << 21, 2048, 1024)
+        LL_ODD(3ull << 14, 384, 128)
+        LL_ODD(3ull << 15, 384, 256)
+        LL_ODD(3ull << 16, 768, 256)
+        LL_ODD(3ull << 17, 768, 512)
+        LL_ODD(3ull << 18, 768, 1024)
+        LL_ODD(3ull << 19, 1536, 1024)
+        LL_ODD(5ull << 13, 320, 128)
+        LL_ODD(5ull << 14, 640, 128)
+        LL_ODD(5ull << 15, 640, 256)
+        LL_ODD(5ull << 16, 640, 512)
+        LL_ODD(5ull << 17, 640, 1024)
+        LL_ODD(5ull << 18, 1280, 1024)
+        LL_ODD(7ull << 13, 448, 128)
+        LL_ODD(7ull << 14, 448, 256)
+        LL_ODD(7ull << 15, 896, 256)
+        LL_ODD(7ull << 16, 896, 512)
+        LL_ODD(7ull << 17, 896, 1024)
+        LL_ODD(7ull << 18, 1792, 1024)
+        LL_ODD(15ull << 13, 960, 128)
+        LL_ODD(15ull << 14, 960, 256)
+        LL_ODD(15ull << 15, 960, 512)
+        LL_ODD(15ull << 16, 960, 1024)
+        LL_ODD(15ull << 17, 1920, 1024)
         default: return false;
     }
 #undef LL_P2
@@ -593,7 +617,7 @@ bool large_launch_for(int transform, u64 M, LargeLaunch &L) {
 }
 
 hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st) {
-    const int W = 1 << L.log_w, H = L.log_h;
+    const int W = L.w_len, H = L.h_len;
     const dim3 gcol(W / kCW), bcol(kCW * (H / kR));
     const dim3 grow(H / 2), brow(2 * (W / kR));
     hipLaunchKernelGGL(L.fwd, gcol, bcol, 0, st, A);
@@ -603,7 +627,7 @@ hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st) 
 }
 
 hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st) {
-    hipLaunchKernelGGL(L.apply, dim3((1 << L.log_w) / kCW), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(L.apply, dim3(L.w_len / kCW), dim3(256), 0, st, A);
     return hipGetLastError();
 }
 

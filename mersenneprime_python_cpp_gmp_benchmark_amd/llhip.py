@@ -59,6 +59,10 @@ def lib():
         L.llh_last_roundoff_trace.argtypes = [P(dbl), ctypes.c_size_t]
         L.llh_supported_lengths.argtypes = [ctypes.c_int, P(u64), ctypes.c_size_t]
         L.llh_supported_lengths.restype = ctypes.c_size_t
+        fl = ctypes.c_float
+        L.llh_mlp_grad.argtypes = [P(fl), P(fl), P(fl), ctypes.c_size_t, P(fl), P(fl)]
+        L.llh_mlp_train.argtypes = [P(fl), P(fl), P(fl), ctypes.c_size_t, ctypes.c_int, fl, P(fl)]
+        L.llh_mlp_predict.argtypes = [P(fl), P(fl), ctypes.c_size_t, P(fl)]
         L.llh_last_roundoff_trace.restype = ctypes.c_size_t
         L.llh_lucas_lehmer.argtypes = [u64, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32), P(i32)]
         _lib = L
@@ -147,3 +151,40 @@ def supported_lengths(transform: int = FP64) -> list[int]:
     out = (ctypes.c_uint64 * n)()
     lib().llh_supported_lengths(transform, out, n)
     return [int(v) for v in out]
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_param_count() -> int:
+    return lib().llh_mlp_param_count()
+
+
+def mlp_grad(params, X, y):
+    """(gradient, loss) of the round-off MLP on the GPU."""
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(mlp_param_count(), np.float32)
+    loss = ctypes.c_float(0)
+    _check(lib().llh_mlp_grad(_fptr(params), _fptr(X), _fptr(y), len(y), _fptr(g), ctypes.byref(loss)))
+    return g, loss.value
+
+
+def mlp_train(params, X, y, epochs: int, lr: float):
+    """(trained params, loss history) -- full-batch Adam on the GPU."""
+    params = np.array(params, np.float32, copy=True)
+    X = np.ascontiguousarray(X, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    hist = np.zeros(epochs, np.float32)
+    _check(lib().llh_mlp_train(_fptr(params), _fptr(X), _fptr(y), len(y), epochs, lr, _fptr(hist)))
+    return params, hist
+
+
+def mlp_predict(params, X):
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros(len(X), np.float32)
+    _check(lib().llh_mlp_predict(_fptr(params), _fptr(X), len(X), _fptr(out)))
+    return out

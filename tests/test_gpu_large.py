@@ -81,7 +81,7 @@ def test_default_length_routes_large_exponents():
 
 
 # lengths with an odd factor (FP64 only): 3, 5, 7, 15 times a power of two, at 14-18 bits/digit
-ODD = [(3 << 15, 1572869), (5 << 14, 1310719), (7 << 14, 1835003), (15 << 13, 2088967),
+ODD = [(3 << 15, 1572869), (5 << 14, 1310719), (7 << 14, 1835003), (15 << 14, 4177927),
        (3 << 18, 12582917), (5 << 17, 10485767), (7 << 17, 16777213), (15 << 16, 16777213),
        (15 << 17, 33554393), (7 << 18, 33554393)]
 

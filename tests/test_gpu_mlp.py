@@ -1,0 +1,64 @@
+"""The round-off MLP on the GPU: gradients and forward pass against the CPU oracle, training
+convergence, and the learned length choice (exactness of runs at the chosen length)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_interim_fixture
+from mersenneprime_python_cpp_gmp_benchmark_amd import llhip, nselect
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    t = (0.5 * X[:, 0] - 0.3 * X[:, 1] ** 2 + np.sin(X[:, 2])).astype(np.float32)
+    return X, t
+
+
+def test_gradients_match_cpu_oracle():
+    X, t = _data(300, 1)
+    P = nselect.init_params(7) + np.random.default_rng(2).normal(0, 0.02, M.N_PARAMS).astype(np.float32)
+    g, loss = llhip.mlp_grad(P, X, t)
+    gref, lref = M.grad(P.astype(np.float64), X.astype(np.float64), t.astype(np.float64))
+    assert abs(loss - lref) <= 1e-4 * max(1.0, lref)
+    scale = np.abs(gref).max()
+    np.testing.assert_allclose(g, gref, rtol=2e-3, atol=2e-5 * scale)
+
+
+def test_forward_matches_cpu_oracle():
+    X, _ = _data(500, 3)
+    P = nselect.init_params(11)
+    y = llhip.mlp_predict(P, X)
+    yref, _ = M.forward(P.astype(np.float64), X.astype(np.float64))
+    np.testing.assert_allclose(y, yref, rtol=1e-4, atol=1e-5)
+
+
+def test_training_converges():
+    X, t = _data(800, 4)
+    P0 = nselect.init_params(0)
+    P, hist = llhip.mlp_train(P0, X, t, 1500, 3e-3)
+    assert hist[-1] < 0.05 * hist[0]
+    # the loss the GPU reports is the oracle's loss at the same parameters
+    assert abs(M.loss(P.astype(np.float64), X, t) - llhip.mlp_grad(P, X, t)[1]) < 1e-4
+
+
+@pytest.mark.skipif(not os.path.exists(nselect.MODEL_PATH), reason="no trained round-off model committed")
+def test_learned_length_is_exact_and_not_longer():
+    """Held-out exponents: the learned length is never longer than the reference's rule, and a
+    run at it (per-iteration check + rollback) gives GMP's Res64."""
+    fx = {(p, k): r for p, k, r in load_interim_fixture("c4_gmp_heldout.csv")}
+    model = nselect.load_model()
+    shorter = 0
+    for (p, k), want in list(fx.items())[:4]:
+        n = nselect.choose_length(p, model)
+        cons = nselect.conservative_length(p)
+        assert n <= cons
+        shorter += n < cons
+        r = llhip.lucas_lehmer_n(p, k, llhip.FP64, n)
+        assert r["status"] == llhip.ST_OK and r["res64"] == want, (p, n, r)
+    assert shorter >= 1
