@@ -48,6 +48,7 @@ struct LargeArgs {
     u32 err_slot;
     u64 n;
     double inv_n;
+    u32 p_mod_n, p_div_n;
 };
 struct LargeLaunch {
     void (*fwd)(LargeArgs);
@@ -474,6 +475,7 @@ static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
     A.wfL = T.wfL; A.wfH = T.wfH; A.wiL = T.wiL; A.wiH = T.wiH;
     A.p = (u32)p; A.log_n = (u32)log_n; A.log_s = T.log_s; A.log_sw = T.log_sw; A.err_slot = 0;
     A.n = N; A.inv_n = 1.0 / (double)N;
+    A.p_mod_n = (u32)(p % N); A.p_div_n = (u32)(p / N);
 
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;

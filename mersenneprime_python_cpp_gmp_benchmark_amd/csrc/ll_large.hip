@@ -102,6 +102,7 @@ struct LargeArgs {
     u32 err_slot;
     u64 n;                            // digit count N
     double inv_n;                     // 1.0 / N
+    u32 p_mod_n, p_div_n;             // p mod N, floor(p / N)
 };
 
 }  // namespace llh
@@ -158,6 +159,30 @@ LL_DEV u32 wexp(u64 d, u32 p, const NDiv &D) {
 }
 LL_HD NDiv ndiv_of(const LargeArgs &A) { return NDiv{A.n, A.inv_n, A.log_n}; }
 
+// Walks digit positions by the residue r_d = d p mod N alone: with P = p mod N, F = p div N,
+//   b_d = F + [r_d + P >= N] + [r_{d+1} > 0] - [r_d > 0],   e_d = (N - r_d) mod N,
+//   r_{d+1} = (r_d + P) mod N,   r_{d+D} = (r_d + (D p mod N)) mod N.
+struct DigitWalk {
+    u32 n, P, F;
+    LL_DEV u32 add(u32 r, u32 delta) const {          // (r + delta) mod N, r, delta < N
+        const u32 v = r + delta;
+        return v >= n ? v - n : v;
+    }
+    LL_DEV u32 next(u32 r) const { return add(r, P); }
+    LL_DEV int bits(u32 r) const {
+        const u32 rn = next(r);
+        return (int)(F + (r + P >= n ? 1u : 0u) + (rn > 0 ? 1u : 0u)) - (r > 0 ? 1 : 0);
+    }
+    LL_DEV u32 wexp(u32 r) const { return r ? n - r : 0u; }
+};
+LL_DEV DigitWalk walk_of(const LargeArgs &A) { return DigitWalk{(u32)A.n, A.p_mod_n, A.p_div_n}; }
+// d p mod N for one position (one exact division; used once per thread or per row)
+LL_DEV u32 rmod(u64 d, u32 p, const NDiv &D) {
+    u64 q, r;
+    divmod_n(d * p, D, q, r);
+    return (u32)r;
+}
+
 // balanced split of v at b bits: v = x + c 2^b, x in [-2^(b-1), 2^(b-1))
 LL_DEV double split_d(double v, int b, double &c) {
     c = floor(__fma_rn(v, ldexp(1.0, -b), 0.5));
@@ -195,16 +220,19 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
 #pragma unroll
     for (int s = 0; s < kR; ++s) lds_all[(t + GH * s) * kCW + c] = x[s];
     __syncthreads();
+    const DigitWalk DW = walk_of(A);
     for (int h = tid; h < H; h += NT) {
         CT cin;
         if (blk > 0) cin = carry[(blk - 1) * H + h];
         else cin = h > 0 ? carry[(NBLK - 1) * H + h - 1] : carry[(NBLK - 1) * H + H - 1];
         T *row = lds_all + h * kCW;
         const u64 j0 = (u64)blk * kCW + (u64)W * h;
+        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             if constexpr (IS_GL) {
-                const int b = bits_of(j0 + cc, p, log_n);
+                const int b = DW.bits(r);
+                r = DW.next(r);
                 i64 v = (i64)row[cc] + cin;
                 if (cc == kCW - 1) { row[cc] = (u64)v; break; }
                 i64 cc2;
@@ -212,8 +240,10 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
                 row[cc] = (u64)v;
                 cin = cc2;
             } else {
-                const u64 d0 = 2 * (j0 + cc);
-                const int b0 = bits_of(d0, p, log_n), b1 = bits_of(d0 + 1, p, log_n);
+                const int b0 = DW.bits(r);
+                r = DW.next(r);
+                const int b1 = DW.bits(r);
+                r = DW.next(r);
                 double c1, c2;
                 const double x0 = split_d(row[cc].re + cin, b0, c1);
                 double x1 = row[cc].im + c1;
@@ -227,19 +257,22 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < kR; ++s) x[s] = lds_all[(t + GH * s) * kCW + c];
-    // ---- weights
+    // ---- weights (digit residues walked down the column: step = W GH elements)
     {
         const int tt = ll_opaque(t);
+        const u64 j0 = (u64)w + (u64)W * tt;
+        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
+        const u32 step = rmod((IS_GL ? 1ull : 2ull) * W * GH, p, log_n);
 #pragma unroll
         for (int s = 0; s < kR; ++s) {
-            const u64 j = (u64)w + (u64)W * (tt + GH * s);
             if constexpr (IS_GL) {
-                const u32 e = wexp(j, p, log_n);
-                x[s] = gl_mul(gl_from_i64((i64)x[s]), tw2<u64>(A.wfL, A.wfH, e, A.log_sw));
+                x[s] = gl_mul(gl_from_i64((i64)x[s]), tw2<u64>(A.wfL, A.wfH, DW.wexp(r), A.log_sw));
             } else {
-                const u32 e0 = wexp(2 * j, p, log_n), e1 = wexp(2 * j + 1, p, log_n);
-                x[s] = cmk(x[s].re * wt2d(A.wfL, A.wfH, e0, A.log_sw), x[s].im * wt2d(A.wfL, A.wfH, e1, A.log_sw));
+                const u32 r1 = DW.next(r);
+                x[s] = cmk(x[s].re * wt2d(A.wfL, A.wfH, DW.wexp(r), A.log_sw),
+                           x[s].im * wt2d(A.wfL, A.wfH, DW.wexp(r1), A.log_sw));
             }
+            r = DW.add(r, step);
         }
     }
     if constexpr (S::MIXED)
@@ -361,20 +394,24 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
         transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, as_global<T>(IS_GL ? A.twHi : A.twH));
     // ---- unweight and round (rows h = t + GH q)
     double eloc = 0.0;
+    const DigitWalk DW = walk_of(A);
     {
         const int tt = ll_opaque(t);
+        const u64 j0 = (u64)w + (u64)W * tt;
+        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
+        const u32 step = rmod((IS_GL ? 1ull : 2ull) * W * GH, p, log_n);
 #pragma unroll
         for (int q = 0; q < kR; ++q) {
-            const u64 j = (u64)w + (u64)W * (tt + GH * q);
+            const u64 j = j0 + (u64)W * GH * q;
+            const u32 rq = r;
+            r = DW.add(r, step);
             if constexpr (IS_GL) {
-                const u32 e = wexp(j, p, log_n);
-                i64 v = gl_to_i64(gl_mul(x[q], tw2<u64>(A.wiL, A.wiH, e, A.log_sw)));
+                i64 v = gl_to_i64(gl_mul(x[q], tw2<u64>(A.wiL, A.wiH, DW.wexp(rq), A.log_sw)));
                 if (j == 0) v -= 2;
                 x[q] = (u64)v;
             } else {
-                const u32 e0 = wexp(2 * j, p, log_n), e1 = wexp(2 * j + 1, p, log_n);
-                const double v0 = x[q].re * wt2d(A.wiL, A.wiH, e0, A.log_sw);
-                const double v1 = x[q].im * wt2d(A.wiL, A.wiH, e1, A.log_sw);
+                const double v0 = x[q].re * wt2d(A.wiL, A.wiH, DW.wexp(rq), A.log_sw);
+                const double v1 = x[q].im * wt2d(A.wiL, A.wiH, DW.wexp(DW.next(rq)), A.log_sw);
                 double r0 = __builtin_rint(v0);
                 const double r1 = __builtin_rint(v1);
                 eloc = fmax(eloc, fmax(fabs(v0 - r0), fabs(v1 - r1)));
@@ -391,19 +428,24 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
     for (int h = tid; h < H; h += NT) {
         T *row = lds_all + h * kCW;
         const u64 j0 = (u64)blk * kCW + (u64)W * h;
+        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
         CT cy = 0;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             if constexpr (IS_GL) {
                 i64 c2;
-                const i64 v = split_i((i64)row[cc] + cy, bits_of(j0 + cc, p, log_n), c2);
+                const i64 v = split_i((i64)row[cc] + cy, DW.bits(r), c2);
+                r = DW.next(r);
                 row[cc] = (u64)v;
                 cy = c2;
             } else {
-                const u64 d0 = 2 * (j0 + cc);
                 double c1, c2;
-                const double x0 = split_d(row[cc].re + cy, bits_of(d0, p, log_n), c1);
-                const double x1 = split_d(row[cc].im + c1, bits_of(d0 + 1, p, log_n), c2);
+                const int b0 = DW.bits(r);
+                r = DW.next(r);
+                const int b1 = DW.bits(r);
+                r = DW.next(r);
+                const double x0 = split_d(row[cc].re + cy, b0, c1);
+                const double x1 = split_d(row[cc].im + c1, b1, c2);
                 row[cc] = cmk(x0, x1);
                 cy = c2;
             }

@@ -113,17 +113,24 @@ def candidate_lengths() -> list[int]:
     return [n for n in llhip.supported_lengths(llhip.FP64) if n >= MIN_LARGE_N]
 
 
-def choose_length(p: int, model: dict | None = None, safe_err: float = SAFE_ERR) -> int:
-    """Smallest supported FP64 length whose predicted round-off is below safe_err, never
+def choose_length(p: int, model: dict | None = None, safe_err: float = SAFE_ERR, costs: dict | None = None) -> int:
+    """The cheapest supported FP64 length whose predicted round-off is below safe_err (cost
+    per squaring from the autotuner's table, shortest length when there is no table), never
     beyond the bits-per-digit range the model was trained on; the conservative length if
-    nothing qualifies."""
+    nothing qualifies.  The conservative length is always a candidate when it is safe."""
     model = model or load_model()
-    cands = [n for n in candidate_lengths() if 14.0 <= p / n <= 21.0]
     cons = conservative_length(p)
-    if not cands:
-        return cons
+    cands = sorted(set(n for n in candidate_lengths() if 14.0 <= p / n <= 21.0) | {cons})
+    if costs is None:
+        try:
+            from .autotune import load_costs
+            costs = load_costs()
+        except (OSError, ValueError, KeyError):
+            costs = {}
     err = predict_err(np.full(len(cands), p), np.array(cands), model)
-    for n, e in sorted(zip(cands, err)):
-        if e <= safe_err:
-            return int(n)
-    return cons
+    safe = [int(n) for n, e in zip(cands, err) if e <= safe_err or n == cons and p / n <= 18.0]
+    if not safe:
+        return cons
+    if costs and all(n in costs for n in safe):
+        return min(safe, key=lambda n: (costs[n], n))
+    return min(safe)
