@@ -116,7 +116,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c3", "c3-ntt", "c2", "p24-fp64", "p24-ntt"], default="c3")
+    ap.add_argument("--workload", choices=["c3", "c3-ntt", "c2", "p24-fp64", "p24-ntt", "copy"], default="c3")
     ap.add_argument("--n", type=int, default=0, help="p24-fp64: transform length (0 = conservative rule)")
     ap.add_argument("--iters", type=int, default=0, help="squarings for the single-exponent workloads")
     ap.add_argument("--cpu-every", type=int, default=200, help="CPU baseline sample: every k-th prime")
@@ -137,6 +137,13 @@ def main() -> int:
     dev = llhip.device_info()
     log(f"device {dev['name']} ({dev['arch']}, {dev['cus']} CUs)")
     interim = load_csv("big_gmp_interim.csv") or {}
+
+    # ---- known-size copy: calibration of the HBM traffic counters -------------------------
+    if a.workload == "copy":
+        nbytes = 1 << 30
+        ms = llhip.device_copy(nbytes, 3)
+        print(json.dumps({"workload": "copy", "bytes": nbytes, "ms": ms, "GBs": 2 * nbytes / ms / 1e6}), flush=True)
+        return 0
 
     # ---- single-exponent workloads (profiling targets) ----------------------------------
     if a.workload in ("c2", "p24-fp64", "p24-ntt"):

@@ -99,6 +99,10 @@ size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap);
  * maximum over the run (lucaslehmer.cpp:289-295). */
 size_t llh_last_roundoff_trace(double *out, size_t cap);
 
+/* Device-to-device copy of `bytes` bytes, `repeats` times; *ms = time per copy.  No reference
+ * counterpart: the known-size transfer the HBM-traffic counters are calibrated against. */
+int llh_device_copy(uint64_t bytes, int repeats, double *ms);
+
 /* ---- round-off model: MLP 3 -> 64 -> 64 -> 1 (FP32, tanh), trained on the GPU ----------------
  * No reference counterpart (the reference fixes N with a bits-per-word cap, lucaslehmer.cpp:246).
  * Inputs are rows of 3 floats (the caller's normalised log2 p, log2 N, bits per digit), targets

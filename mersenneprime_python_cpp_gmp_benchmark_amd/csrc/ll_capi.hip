@@ -814,6 +814,36 @@ size_t llh_last_roundoff_trace(double *out, size_t cap) {
     return n;
 }
 
+int llh_device_copy(uint64_t bytes, int repeats, double *ms) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (bytes == 0 || repeats < 1) {
+        set_error("device_copy: bytes and repeats must be positive");
+        return LLH_ERR_ARG;
+    }
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, bytes));
+    HIPCHK(hipMalloc(&b, bytes));
+    hipStream_t st = g_dev.stream;
+    HIPCHK(hipMemsetAsync(a, 1, bytes, st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    for (int i = 0; i < repeats; ++i) HIPCHK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return LLH_OK;
+}
+
 int llh_mlp_param_count(void) { return mlp_n_params(); }
 
 // shared set-up of the MLP entry points: buffers sized for n samples, parameters and data

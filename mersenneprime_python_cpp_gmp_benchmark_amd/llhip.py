@@ -59,6 +59,7 @@ def lib():
         L.llh_last_roundoff_trace.argtypes = [P(dbl), ctypes.c_size_t]
         L.llh_supported_lengths.argtypes = [ctypes.c_int, P(u64), ctypes.c_size_t]
         L.llh_supported_lengths.restype = ctypes.c_size_t
+        L.llh_device_copy.argtypes = [u64, ctypes.c_int, P(dbl)]
         fl = ctypes.c_float
         L.llh_mlp_grad.argtypes = [P(fl), P(fl), P(fl), ctypes.c_size_t, P(fl), P(fl)]
         L.llh_mlp_train.argtypes = [P(fl), P(fl), P(fl), ctypes.c_size_t, ctypes.c_int, fl, P(fl)]
@@ -188,3 +189,10 @@ def mlp_predict(params, X):
     out = np.zeros(len(X), np.float32)
     _check(lib().llh_mlp_predict(_fptr(params), _fptr(X), len(X), _fptr(out)))
     return out
+
+
+def device_copy(nbytes: int, repeats: int = 1) -> float:
+    """Milliseconds per device-to-device copy of nbytes (traffic-counter calibration)."""
+    ms = ctypes.c_double(0)
+    _check(lib().llh_device_copy(nbytes, repeats, ctypes.byref(ms)))
+    return ms.value
