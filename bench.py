@@ -280,7 +280,7 @@ def main() -> int:
         "data": "synthetic (deterministic: the exponent list; s_0 = 4)",
         "config": {"workload": f"batched full Lucas-Lehmer of all {len(ps)} primes p <= 100001 in one "
                                f"persistent launch ({'fp64' if tr == llhip.FP64 else 'ntt'} IBDWT)",
-                   "exponents": len(ps), "squarings": total_iters, "bits_per_word_cap": llhip.get_bpw()},
+                   "exponents": len(ps), "squarings": total_iters, "bits_per_word_cap": llhip.get_batch_bpw()},
         "res64_diffs_vs_gmp": diffs, "res64_checked": checked, "gmp_fixture_partial": fx_partial,
         "not_ok_status": bad_status,
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,

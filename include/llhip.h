@@ -55,6 +55,10 @@ uint64_t llh_fft_length(uint64_t p, double max_bpw);
  * For LLH_NTT this extends ntt_length cuda/ll_ntt.cu:176-180 down to N = 16. */
 uint64_t llh_batch_length(uint64_t p, int transform);
 
+/* Smallest FP64 length the engines run (batched powers of two, then the large engine's
+ * {1,3,5,7,15} * 2^k) with p/N <= max_bpw (<= 0: the bits-per-word knob).  0 = none. */
+uint64_t llh_fp64_length(uint64_t p, double max_bpw);
+
 /* Reference NTT length rule (N in 64..8192, 0 = too large).  cuda/ll_ntt.cu:176-180. */
 uint64_t llh_ntt_length(uint64_t p);
 
@@ -65,6 +69,12 @@ uint64_t llh_max_exponent(int transform);
  * Replaces set_fft_bits_per_word lucaslehmer.cpp:368-369 (atomic here, not a plain static). */
 int llh_set_fft_bits_per_word(double bits);
 double llh_get_fft_bits_per_word(void);
+
+/* Bits-per-digit cap of the batched FP64 engine's power-of-two lengths (default 20: its worst
+ * round-off over every prime <= 100001 is 0.053).  No reference counterpart; the reference uses
+ * one cap for every length. */
+int llh_set_batch_bits_per_word(double bits);
+double llh_get_batch_bits_per_word(void);
 
 /* Many exponents in one persistent launch.  Replaces lucas_lehmer_batch cuda/ll_ntt.cu:259-267
  * (and the per-thread engine calls of PerfNumMultiCLL.py:189-193).  max_iters = 0 runs the full

@@ -93,7 +93,11 @@ static std::map<std::pair<int, u64>, LargeTables> g_large_tables;
 static std::vector<double> g_err_trace;     // per-iteration max round-off of the last large run
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
-static std::atomic<double> g_bpw{18.0};
+static std::atomic<double> g_bpw{18.0};        // reference knob: the conservative length rule
+// Bits-per-digit cap of the batched FP64 engine (power-of-two lengths up to 8192).  Its worst
+// round-off over every prime <= 100001 is 0.053 at 20 bits/digit (0.003 at 18), far below the
+// 0.4 rollback threshold, so short lengths take more bits than the reference's 18.
+static std::atomic<double> g_batch_bpw{20.0};
 
 // batched engine limits
 static const int kFp64MinLog = 5, kFp64MaxLog = 13;   // N = 32 .. 8192 (M = 16 .. 4096)
@@ -229,7 +233,7 @@ static int batch_log_n(u64 p, int transform) {
     // the kernels index digits with 32-bit products j * p, j < N
     auto fits = [p](int l) { return (((u64)1 << l) + 2) * p < ((u64)1 << 32); };
     if (transform == kFP64) {
-        const double bpw = g_bpw.load();
+        const double bpw = g_batch_bpw.load();
         for (int l = kFp64MinLog; l <= kFp64MaxLog; ++l)
             if ((double)p / (double)((u64)1 << l) <= bpw) return fits(l) ? l : 0;
         return 0;
@@ -739,9 +743,21 @@ uint64_t llh_batch_length(uint64_t p, int transform) {
 
 uint64_t llh_ntt_length(uint64_t p) { return ntt_length_ref(p); }
 
+uint64_t llh_fp64_length(uint64_t p, double max_bpw) {
+    if (p < 64) return 0;
+    if (!(max_bpw > 0)) max_bpw = g_bpw.load();
+    for (int l = kFp64MinLog; l <= kFp64MaxLog; ++l) {
+        const u64 N = (u64)1 << l;
+        if ((double)p / (double)N <= max_bpw && (N + 2) * p < ((u64)1 << 32)) return N;
+    }
+    for (u64 N : large_lengths(kFP64))
+        if ((double)p / (double)N <= max_bpw) return N;
+    return 0;
+}
+
 uint64_t llh_max_exponent(int transform) {
     if (transform == kNTT) return ((uint64_t)1 << kNttMaxLog) * (uint64_t)ntt_max_bits((uint64_t)1 << kNttMaxLog);
-    return (uint64_t)std::floor(g_bpw.load() * (double)((uint64_t)1 << kFp64MaxLog));
+    return (uint64_t)std::floor(g_batch_bpw.load() * (double)((uint64_t)1 << kFp64MaxLog));
 }
 
 int llh_set_fft_bits_per_word(double bits) {
@@ -754,6 +770,17 @@ int llh_set_fft_bits_per_word(double bits) {
 }
 
 double llh_get_fft_bits_per_word(void) { return g_bpw.load(); }
+
+int llh_set_batch_bits_per_word(double bits) {
+    if (!(bits > 0) || bits > 64) {
+        set_error("bits per word must be in (0, 64]");
+        return LLH_ERR_ARG;
+    }
+    g_batch_bpw.store(bits);
+    return LLH_OK;
+}
+
+double llh_get_batch_bits_per_word(void) { return g_batch_bpw.load(); }
 
 int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, int32_t *verdict, uint64_t *res64,
               double *seconds, double *maxerr, uint32_t *n_used, int32_t *status) {

@@ -40,13 +40,17 @@ void check(int rc) {
     if (rc != LLH_OK) throw std::runtime_error(std::string("llhip: ") + llh_last_error());
 }
 
+// One exponent.  The FP64 engine runs at the smallest length within the module's
+// bits-per-word knob (set_fft_bits_per_word, default 18, lucaslehmer.cpp:246); it still rolls
+// back to a longer length if the round-off check fires.
 Run run_one(unsigned long p, unsigned long iters, int transform) {
     Run r;
-    uint64_t pp = p;
     int rc;
     {
         py::gil_scoped_release release;
-        rc = llh_batch(&pp, 1, iters, transform, &r.verdict, &r.res64, &r.seconds, &r.maxerr, &r.n, &r.status);
+        const uint64_t n = transform == LLH_FP64 ? llh_fp64_length(p, 0.0) : 0;
+        rc = llh_lucas_lehmer_n(p, iters, transform, n, 0, &r.verdict, &r.res64, &r.maxerr, &r.n, &r.status,
+                                &r.seconds);
     }
     check(rc);
     return r;

@@ -51,6 +51,8 @@ def lib():
         L.llh_max_exponent.restype = u64
         L.llh_set_fft_bits_per_word.argtypes = [dbl]
         L.llh_get_fft_bits_per_word.restype = dbl
+        L.llh_set_batch_bits_per_word.argtypes = [dbl]
+        L.llh_get_batch_bits_per_word.restype = dbl
         L.llh_batch.argtypes = [P(u64), ctypes.c_size_t, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(dbl),
                                 P(u32), P(i32)]
         L.llh_last_launch_stats.argtypes = [P(dbl), P(u32)]
@@ -90,6 +92,14 @@ def set_bpw(bits: float) -> None:
 
 def get_bpw() -> float:
     return lib().llh_get_fft_bits_per_word()
+
+
+def set_batch_bpw(bits: float) -> None:
+    _check(lib().llh_set_batch_bits_per_word(bits))
+
+
+def get_batch_bpw() -> float:
+    return lib().llh_get_batch_bits_per_word()
 
 
 def batch(ps, iters: int = 0, transform: int = NTT) -> dict:

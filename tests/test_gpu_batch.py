@@ -74,13 +74,13 @@ def test_forced_rollback_stays_exact():
     to longer lengths and still return GMP's Res64."""
     rows = load_range_fixture("c1_gmp_p10001.csv")
     sel = [r for r in rows if r[0] > 3000][::40]
-    old = llhip.get_bpw()
+    old = llhip.get_batch_bpw()
     try:
-        llhip.set_bpw(40.0)
+        llhip.set_batch_bpw(40.0)
         first = np.array([llhip.lib().llh_batch_length(r[0], llhip.FP64) for r in sel])
         out = llhip.batch([r[0] for r in sel], 0, llhip.FP64)
     finally:
-        llhip.set_bpw(old)
+        llhip.set_batch_bpw(old)
     _check_rows(sel, out)
     assert (out["n_used"] > first).any()          # some exponents were rolled back to 2N
     assert (out["n_used"] >= first).all()
