@@ -34,31 +34,9 @@ hipError_t launch_batch(int transform, const ClassDesc *d_classes, int ncls, con
                         BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream);
 int batch_blocks_per_cu(int transform);
 
-struct LargeArgs {
-    void *elem;
-    void *carry;
-    unsigned long long *err;
-    const void *twW, *twWi;
-    const void *twH, *twHi;
-    const void *tmL, *tmH;
-    const void *tmLi, *tmHi;
-    const void *wfL, *wfH;
-    const void *wiL, *wiH;
-    u32 p, log_n, log_s, log_sw;
-    u32 err_slot;
-    u64 n;
-    double inv_n;
-    u32 p_mod_n, p_div_n;
-};
-struct LargeLaunch {
-    void (*fwd)(LargeArgs);
-    void (*rows)(LargeArgs);
-    void (*inv)(LargeArgs);
-    void (*apply)(LargeArgs);
-    int w_len, h_len;
-};
 bool large_launch_for(int transform, u64 M, LargeLaunch &L);
-hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
+int large_blocks(const LargeLaunch &L);
+hipError_t large_squarings(const LargeLaunch &L, LargeArgs A, int k, hipStream_t st);
 hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
 hipError_t large_normalize(int transform, const void *elem, i64 *dig, const LargeArgs &A, void *out, hipStream_t st);
 
@@ -455,16 +433,22 @@ static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
     const u64 H = (u64)L.h_len, W = (u64)L.w_len;
     const int log_n = (N & (N - 1)) ? 0 : ceil_log2(N);
     const size_t esz = transform == kFP64 ? 16 : 8;
-    const u64 nblk = W / 4;
-    void *elem = nullptr, *carry = nullptr, *dig = nullptr, *errs = nullptr, *nout = nullptr;
+    const u64 nblk = (u64)large_blocks(L);
+    // errs: kGraphBlock round-off slots, then the hand-off fault word
+    const size_t errs_bytes = (kGraphBlock + 1) * 8;
+    // sync: the fused pass's launch counter, then one flag per column block
+    const size_t sync_bytes = 8 + nblk * 4;
+    void *elem = nullptr, *carry = nullptr, *dig = nullptr, *errs = nullptr, *nout = nullptr, *sync = nullptr;
     HIPCHK(hipMalloc(&elem, M * esz));
     HIPCHK(hipMalloc(&carry, nblk * H * 8));
     HIPCHK(hipMalloc(&dig, N * 8));
-    HIPCHK(hipMalloc(&errs, kGraphBlock * 8));
+    HIPCHK(hipMalloc(&errs, errs_bytes));
     HIPCHK(hipMalloc(&nout, sizeof(NormOutH)));
+    HIPCHK(hipMalloc(&sync, sync_bytes));
     hipStream_t st = g_dev.stream;
     HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
     HIPCHK(hipMemsetAsync(carry, 0, nblk * H * 8, st));
+    HIPCHK(hipMemsetAsync(sync, 0, sync_bytes, st));
     if (transform == kFP64) {
         const double four = 4.0;
         HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
@@ -480,10 +464,13 @@ static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
     A.p = (u32)p; A.log_n = (u32)log_n; A.log_s = T.log_s; A.log_sw = T.log_sw; A.err_slot = 0;
     A.n = N; A.inv_n = 1.0 / (double)N;
     A.p_mod_n = (u32)(p % N); A.p_div_n = (u32)(p / N);
+    A.ticket = (unsigned long long *)sync;
+    A.flags = (unsigned *)((char *)sync + 8);
+    A.fault = (unsigned *)((char *)errs + kGraphBlock * 8);
 
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    std::vector<double> eh(kGraphBlock);
+    std::vector<double> eh(kGraphBlock + 1);
     g_err_trace.clear();
     g_err_trace.reserve(iters);
     hipEvent_t ev0, ev1;
@@ -493,34 +480,36 @@ static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
     r.status = kStOk;
     r.maxerr = 0;
     u64 done = 0;
+    bool fault = false;
     while (done < iters) {
         const u64 k = std::min<u64>(kGraphBlock, iters - done);
-        HIPCHK(hipMemsetAsync(errs, 0, kGraphBlock * 8, st));
+        HIPCHK(hipMemsetAsync(errs, 0, errs_bytes, st));
         if (k == (u64)kGraphBlock) {
             if (!gexec) {
                 HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                for (u64 i = 0; i < k; ++i) {
-                    A.err_slot = (u32)i;
-                    hipError_t e = large_step(L, A, st);
-                    if (e != hipSuccess) {
-                        hipGraph_t g2;
-                        (void)hipStreamEndCapture(st, &g2);
-                        set_error(std::string("HIP: ") + hipGetErrorString(e) + " while capturing");
-                        return LLH_ERR_HIP;
-                    }
+                hipError_t e = large_squarings(L, A, (int)k, st);
+                if (e != hipSuccess) {
+                    hipGraph_t g2;
+                    (void)hipStreamEndCapture(st, &g2);
+                    set_error(std::string("HIP: ") + hipGetErrorString(e) + " while capturing");
+                    return LLH_ERR_HIP;
                 }
                 HIPCHK(hipStreamEndCapture(st, &graph));
                 HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
             }
             HIPCHK(hipGraphLaunch(gexec, st));
         } else {
-            for (u64 i = 0; i < k; ++i) {
-                A.err_slot = (u32)i;
-                HIPCHK(large_step(L, A, st));
-            }
+            HIPCHK(large_squarings(L, A, (int)k, st));
         }
-        HIPCHK(hipMemcpyAsync(eh.data(), errs, kGraphBlock * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(eh.data(), errs, errs_bytes, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        unsigned fw;
+        std::memcpy(&fw, &eh[kGraphBlock], sizeof fw);
+        if (fw) {
+            fault = true;
+            r.status = kStNotRun;
+            break;
+        }
         double bmax = 0;
         for (u64 i = 0; i < k; ++i) {
             g_err_trace.push_back(eh[i]);
@@ -562,6 +551,11 @@ static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
     (void)hipFree(dig);
     (void)hipFree(errs);
     (void)hipFree(nout);
+    (void)hipFree(sync);
+    if (fault) {
+        set_error("large engine: a carry hand-off between workgroups timed out");
+        return LLH_ERR_HIP;
+    }
     return LLH_OK;
 }
 

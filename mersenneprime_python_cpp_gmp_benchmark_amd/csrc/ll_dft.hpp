@@ -19,6 +19,14 @@
 
 LL_HD constexpr int lds_pad(int i) { return i + (i >> 4); }
 
+// Twiddle table access: a global table (gptr) or a copy staged in LDS (generic pointer to
+// __shared__ memory, which the compiler lowers to ds_read once the kernel is inlined).
+template <class W, class TP>
+LL_DEV W tw_at(TP tw, u32 idx) {
+    if constexpr (std::is_same<TP, gptr<W>>::value) return ld_table(tw, idx);
+    else return tw[idx];
+}
+
 template <class F, bool INV>
 LL_DEV void dft2(typename F::T &a0, typename F::T &a1) {
     typename F::T t = F::sub(a0, a1);
@@ -158,9 +166,8 @@ LL_DEV void dft_r(typename F::T *x) {
 // threads; t is the thread's index inside its group.  `tw` holds w_M^j, j < M (for the
 // inverse transform the C64 path conjugates, the GL path is handed the inverse table).
 // FIRST: inputs come from registers;  LAST: outputs stay in registers.
-template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV>
-LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t,
-                          gptr<typename F::W> tw) {
+template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV, class TP>
+LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(R % r == 0, "radix must divide the per-thread count");
     static_assert(!FIRST || r == R, "first pass must use the full per-thread radix");
     static_assert(!LAST || (r == R && L * r == M), "last pass must use the full radix");
@@ -182,7 +189,7 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t,
             constexpr int TSTEP = M / (r * L);
 #pragma unroll
             for (int s = 1; s < r; ++s)
-                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], ld_table(tw, (u32)(s * k * TSTEP)));
+                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], tw_at<typename F::W>(tw, (u32)(s * k * TSTEP)));
         }
     }
 #pragma unroll
@@ -200,9 +207,8 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t,
 }
 
 // Whole transform: two, three or four passes (R3 == 0: two; R4 == 0: three).
-template <class F, int M, int G, int R, int R1, int R2, int R3, int R4, bool INV>
-LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t,
-                      gptr<typename F::W> tw) {
+template <class F, int M, int G, int R, int R1, int R2, int R3, int R4, bool INV, class TP>
+LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     if constexpr (R4 != 0) {
         static_assert(R1 * R2 * R3 * R4 == M, "radices must multiply to M");
         stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
@@ -226,8 +232,8 @@ LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t,
 // Used where the register-blocked form does not apply (lengths with an odd factor).  The data
 // sits in LDS in natural order (padded); every pass reads its butterflies' inputs, barriers,
 // and writes the outputs in place.  A thread runs ceil((H/r)/G) butterflies per pass.
-template <class F, int H, int G, int r, int L, bool INV>
-LL_DEV void mixed_pass(typename F::T *lds, int t, gptr<typename F::W> tw) {
+template <class F, int H, int G, int r, int L, bool INV, class TP>
+LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
     typedef typename F::T T;
     constexpr int NBF = H / r;
     constexpr int NB = (NBF + G - 1) / G;
@@ -246,7 +252,7 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, gptr<typename F::W> tw) {
             if constexpr (L > 1) {
 #pragma unroll
                 for (int s = 1; s < r; ++s)
-                    y[b][s] = F::template twiddle<INV>(y[b][s], ld_table(tw, (u32)(s * k * TSTEP)));
+                    y[b][s] = F::template twiddle<INV>(y[b][s], tw_at<typename F::W>(tw, (u32)(s * k * TSTEP)));
             }
             dft_r<F, r, INV>(y[b]);
         }
@@ -264,8 +270,8 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, gptr<typename F::W> tw) {
 }
 
 // x (column layout: thread t holds elements t + G s, s < R) -> transform -> x (same layout)
-template <class F, int H, int G, int R, bool INV, int R1, int R2, int R3, int R4>
-LL_DEV void mixed_transform(typename F::T (&x)[R], typename F::T *lds, int t, gptr<typename F::W> tw) {
+template <class F, int H, int G, int R, bool INV, int R1, int R2, int R3, int R4, class TP>
+LL_DEV void mixed_transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(G * R == H, "column layout must cover the length");
     static_assert(R1 * (R2 ? R2 : 1) * (R3 ? R3 : 1) * (R4 ? R4 : 1) == H, "radices must multiply to H");
     const int tt = ll_opaque(t);
@@ -286,8 +292,8 @@ LL_DEV void mixed_transform(typename F::T (&x)[R], typename F::T *lds, int t, gp
 // For a length W = 8 * m * 8 (m = product of M1 M2 M3, any of 2..16 including 3, 5, 7):
 // the first radix-8 pass reads the column layout from registers, the middle passes run
 // through LDS (mixed_pass), the last radix-8 pass leaves the column layout in registers.
-template <class F, int W, int G, int R, bool INV, int M1, int M2, int M3>
-LL_DEV void hybrid_transform(typename F::T (&x)[R], typename F::T *lds, int t, gptr<typename F::W> tw) {
+template <class F, int W, int G, int R, bool INV, int M1, int M2, int M3, class TP>
+LL_DEV void hybrid_transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(R == 8 && G * R == W, "hybrid transform runs 8 elements per thread");
     static_assert(8 * M1 * (M2 ? M2 : 1) * (M3 ? M3 : 1) * 8 == W, "radices must multiply to W");
     stockham_pass<F, W, G, R, 8, 1, true, false, INV>(x, lds, t, tw);

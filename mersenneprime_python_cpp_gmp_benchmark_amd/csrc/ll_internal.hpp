@@ -53,6 +53,37 @@ struct ClassDesc {
     const void *wi;     // inverse weights incl. the transform's 1/M or 1/N scale
 };
 
+// ---- large engine (ll_large.hip) ----------------------------------------------------------
+// Kernel arguments (by value).  Table pointers are device addresses.
+struct LargeArgs {
+    void *elem;              // [M] elements
+    void *carry;             // [nblk][H] row-segment carry-outs (double or i64)
+    unsigned long long *err; // per-iteration max round-off slots (double bits)
+    const void *twW, *twWi;  // w_W^j, j < W   (GL: forward, inverse)
+    const void *twH, *twHi;  // w_H^j, j < H
+    const void *tmL, *tmH;   // w_M^e = tmL[e & (S-1)] * tmH[e >> s]
+    const void *tmLi, *tmHi; // GL inverse of the above
+    const void *wfL, *wfH;   // forward weight of exponent e: wfL[e & (Sw-1)] * wfH[e >> sw]
+    const void *wiL, *wiH;   // inverse weight (with the transform's 1/M or 1/N)
+    u32 p, log_n, log_s, log_sw;     // log_n: log2 N when N is a power of two, else 0
+    u32 err_slot;
+    u64 n;                            // digit count N
+    double inv_n;                     // 1.0 / N
+    u32 p_mod_n, p_div_n;             // p mod N, floor(p / N)
+    unsigned long long *ticket;       // fused column pass: launch counter (only grows)
+    unsigned *flags;                  // [W / 4] per-block carry-out sequence numbers
+    unsigned *fault;                  // raised when a carry hand-off wait gives up
+};
+
+struct LargeLaunch {
+    void (*fwd)(LargeArgs);
+    void (*rows)(LargeArgs);
+    void (*inv)(LargeArgs);
+    void (*fused)(LargeArgs);
+    void (*apply)(LargeArgs);
+    int w_len, h_len;   // W and H (either may carry the odd factor)
+};
+
 // ---- host helpers (ll_tables.cpp) ---------------------------------------------------------
 u64 fft_length_conservative(u64 p, double max_bpw);   // lucaslehmer.cpp:244-254
 u64 ntt_length_ref(u64 p);                            // cuda/ll_ntt.cu:176-180 (extended)

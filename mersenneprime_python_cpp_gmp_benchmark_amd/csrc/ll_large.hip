@@ -85,27 +85,6 @@ struct ElemOf {
 
 }  // namespace
 
-namespace llh {
-
-// Kernel arguments (by value).  Table pointers are device addresses.
-struct LargeArgs {
-    void *elem;              // [M] elements
-    void *carry;             // [nblk][H] row-segment carry-outs (double or i64)
-    unsigned long long *err; // per-iteration max round-off slots (double bits)
-    const void *twW, *twWi;  // w_W^j, j < W   (GL: forward, inverse)
-    const void *twH, *twHi;  // w_H^j, j < H
-    const void *tmL, *tmH;   // w_M^e = tmL[e & (S-1)] * tmH[e >> s]
-    const void *tmLi, *tmHi; // GL inverse of the above
-    const void *wfL, *wfH;   // forward weight of exponent e: wfL[e & (Sw-1)] * wfH[e >> sw]
-    const void *wiL, *wiH;   // inverse weight (with the transform's 1/M or 1/N)
-    u32 p, log_n, log_s, log_sw;     // log_n: log2 N when N is a power of two, else 0
-    u32 err_slot;
-    u64 n;                            // digit count N
-    double inv_n;                     // 1.0 / N
-    u32 p_mod_n, p_div_n;             // p mod N, floor(p / N)
-};
-
-}  // namespace llh
 
 namespace {
 
@@ -193,108 +172,414 @@ LL_DEV i64 split_i(i64 v, int b, i64 &c) {
     return v - (c << b);
 }
 
-// ---- K1 -----------------------------------------------------------------------------------
+// ---- cross-workgroup hand-off ---------------------------------------------------------------
+// Values another workgroup reads inside the same launch are written and read with agent-scope
+// atomics, which go to the coherence point shared by all XCDs (the per-XCD L2s are not coherent
+// with each other inside a launch).  A workgroup publishes by completing those stores
+// (s_waitcnt vmcnt(0)), a barrier, then one flag store; a reader polls the flag, then reads.
+LL_DEV void st_coh(u64 *p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+LL_DEV u64 ld_coh_u(const u64 *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+LL_DEV void st_coh(double *p, double v) { st_coh((u64 *)p, (u64)__double_as_longlong(v)); }
+LL_DEV void st_coh(i64 *p, i64 v) { st_coh((u64 *)p, (u64)v); }
+LL_DEV void ld_coh(const double *p, double &v) { v = __longlong_as_double((long long)ld_coh_u((const u64 *)p)); }
+LL_DEV void ld_coh(const i64 *p, i64 &v) { v = (i64)ld_coh_u((const u64 *)p); }
+
+// wait for this thread's vector-memory operations only (vmcnt = 0; expcnt and lgkmcnt at
+// their "no wait" maxima, gfx9 encoding)
+LL_DEV void wait_vmem() { __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8)); }
+
+LL_DEV void publish_flag(unsigned *flag, u32 seq, int tid) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread 0 polls until flag >= seq (sequence numbers, wrap-safe compare), for at most about
+// two seconds of wall clock; past that it raises *fault and stops waiting, so the launch always
+// drains (and once *fault is up, later waits give up at once).  The whole workgroup leaves
+// together.
+constexpr u64 kSpinTicks = 200000000ull;   // 2 s at the usual 100 MHz wall clock
+LL_DEV void wait_flag(const unsigned *flag, u32 seq, unsigned *fault, int tid) {
+    if (tid == 0) {
+        const u64 t0 = wall_clock64();
+        while ((int)(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - seq) < 0) {
+            if (wall_clock64() - t0 > kSpinTicks ||
+                __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                atomicOr(fault, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    wait_vmem();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __syncthreads();
+}
+
+// Copy a LEN-entry twiddle table into LDS (all NT threads of the workgroup).
+template <class T, int LEN, int NT>
+LL_DEV void stage_table(T *dst, const void *src, int tid) {
+    const gptr<T> s = as_global<T>(src);
+#pragma unroll
+    for (int i = 0; i < (LEN + NT - 1) / NT; ++i) {
+        const int k = tid + NT * i;
+        if (LEN % NT == 0 || k < LEN) dst[k] = ld_table(s, (u32)k);
+    }
+}
+
+// ---- column kernels ---------------------------------------------------------------------------
+// A workgroup owns a block of kCW adjacent columns (W / kCW blocks).  Thread (c, t), with
+// c = tid % kCW, holds column w = blk kCW + c at rows t + GH q: the column layout of the H-point
+// transform.  Between the inverse and the forward transform the digits sit in LDS in the
+// row-segment layout lds_all[h kCW + c] (a row segment is kCW elements, 2 kCW digits for FP64,
+// contiguous digits), and thread tid walks the segments of rows h = tid + NT i.
+constexpr int kNRow = kR / kCW;                              // row segments per thread
+LL_HD constexpr int nwt(bool gl) { return gl ? kR : 2 * kR; }  // digit weights per thread
+
 template <bool IS_GL, int W, int H>
-__global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
+struct Col {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
     typedef typename E::CT CT;
+    typedef typename E::WT WT;
     typedef ColSched<H> S;
-    constexpr int GH = H / kR, NT = kCW * GH;
-    constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
-    constexpr int NBLK = W / kCW;
-    __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
-    const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
-    const int blk = blockIdx.x, w = blk * kCW + c;
-    T *lds = lds_all + c * REG;
-    T *elem = reinterpret_cast<T *>(A.elem);
-    const CT *carry = reinterpret_cast<const CT *>(A.carry);
-    const u32 p = A.p;
-    const NDiv log_n = ndiv_of(A);
+    static constexpr int GH = H / kR, NT = kCW * GH, NBLK = W / kCW;
+    static constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
+    static constexpr int NROW = kNRow;
+    static constexpr int NWT = nwt(IS_GL);
+    static_assert(H == NT * NROW, "row segments must split evenly over the workgroup");
+};
 
-    T x[kR];
+// w_M^{w k_h} at the thread's positions k_h = t + GH q (tables lo/hi: forward or GL inverse)
+template <bool IS_GL, int W, int H>
+LL_DEV void col_twiddles(typename Col<IS_GL, W, H>::T (&tm)[kR], const void *lo, const void *hi, u32 log_s,
+                         int w, int t) {
+    typedef Col<IS_GL, W, H> C;
 #pragma unroll
-    for (int s = 0; s < kR; ++s) x[s] = elem[w + W * (t + GH * s)];
-    // ---- carry-in: transpose to rows, add the previous segment's carry-out, re-split
+    for (int q = 0; q < kR; ++q) tm[q] = tw2<typename C::T>(lo, hi, (u32)w * (u32)(t + C::GH * q), log_s);
+}
+
+// Digit weights (tables lo/hi: forward or inverse) at the thread's elements
+// j = w + W (t + GH q); FP64 elements hold digits 2j and 2j + 1.
+template <bool IS_GL, int W, int H>
+LL_DEV void col_weights(typename Col<IS_GL, W, H>::WT (&wt)[nwt(IS_GL)], const void *lo,
+                        const void *hi, const LargeArgs &A, int w, int t) {
+    typedef Col<IS_GL, W, H> C;
+    const DigitWalk DW = walk_of(A);
+    const NDiv D = ndiv_of(A);
+    const u64 j0 = (u64)w + (u64)W * t;
+    u32 r = rmod(IS_GL ? j0 : 2 * j0, A.p, D);
+    const u32 step = rmod((IS_GL ? 1ull : 2ull) * W * C::GH, A.p, D);
 #pragma unroll
-    for (int s = 0; s < kR; ++s) lds_all[(t + GH * s) * kCW + c] = x[s];
+    for (int q = 0; q < kR; ++q) {
+        if constexpr (IS_GL) {
+            wt[q] = tw2<u64>(lo, hi, DW.wexp(r), A.log_sw);
+        } else {
+            wt[2 * q] = wt2d(lo, hi, DW.wexp(r), A.log_sw);
+            wt[2 * q + 1] = wt2d(lo, hi, DW.wexp(DW.next(r)), A.log_sw);
+        }
+        r = DW.add(r, step);
+    }
+}
+
+// x: column layout, already multiplied by the inverse w_M twiddles.  Inverse H-point transform,
+// unweight, round (max round-off into eloc), -2 at digit 0, then the row-segment layout in LDS
+// with an exact balanced carry along each segment; cy[i] is the carry out of row tid + NT i.
+template <bool IS_GL, int W, int H, class TP>
+LL_DEV void col_inverse(typename Col<IS_GL, W, H>::T (&x)[kR], typename Col<IS_GL, W, H>::T *lds_all, TP tw,
+                        const typename Col<IS_GL, W, H>::WT (&wi)[nwt(IS_GL)], const LargeArgs &A,
+                        int blk, int c, int t, int tid,
+                        typename Col<IS_GL, W, H>::CT (&cy)[kNRow], double &eloc) {
+    typedef Col<IS_GL, W, H> C;
+    typedef typename C::F F;
+    typedef typename C::T T;
+    typedef typename C::CT CT;
+    typedef typename C::S S;
+    T *lds = lds_all + c * C::REG;
+    if constexpr (S::MIXED)
+        mixed_transform<F, H, C::GH, kR, true, S::R1, S::R2, S::R3, S::R4>(x, lds, t, tw);
+    else
+        transform<F, H, C::GH, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, tw);
+    const bool has_d0 = blk == 0 && c == 0 && t == 0;
+#pragma unroll
+    for (int q = 0; q < kR; ++q) {
+        if constexpr (IS_GL) {
+            i64 v = gl_to_i64(gl_mul(x[q], wi[q]));
+            if (q == 0 && has_d0) v -= 2;
+            x[q] = (u64)v;
+        } else {
+            const double v0 = x[q].re * wi[2 * q];
+            const double v1 = x[q].im * wi[2 * q + 1];
+            double r0 = __builtin_rint(v0);
+            const double r1 = __builtin_rint(v1);
+            eloc = fmax(eloc, fmax(fabs(v0 - r0), fabs(v1 - r1)));
+            if (q == 0 && has_d0) r0 -= 2.0;
+            x[q] = cmk(r0, r1);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kR; ++q) lds_all[(t + C::GH * q) * kCW + c] = x[q];
     __syncthreads();
     const DigitWalk DW = walk_of(A);
-    for (int h = tid; h < H; h += NT) {
-        CT cin;
-        if (blk > 0) cin = carry[(blk - 1) * H + h];
-        else cin = h > 0 ? carry[(NBLK - 1) * H + h - 1] : carry[(NBLK - 1) * H + H - 1];
+    const NDiv D = ndiv_of(A);
+#pragma unroll
+    for (int i = 0; i < C::NROW; ++i) {
+        const int h = tid + C::NT * i;
         T *row = lds_all + h * kCW;
         const u64 j0 = (u64)blk * kCW + (u64)W * h;
-        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
+        u32 r = rmod(IS_GL ? j0 : 2 * j0, A.p, D);
+        CT cv = 0;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) {
+            if constexpr (IS_GL) {
+                i64 c2;
+                const i64 v = split_i((i64)row[cc] + cv, DW.bits(r), c2);
+                r = DW.next(r);
+                row[cc] = (u64)v;
+                cv = c2;
+            } else {
+                double c1, c2;
+                const int b0 = DW.bits(r);
+                r = DW.next(r);
+                const int b1 = DW.bits(r);
+                r = DW.next(r);
+                const double x0 = split_d(row[cc].re + cv, b0, c1);
+                const double x1 = split_d(row[cc].im + c1, b1, c2);
+                row[cc] = cmk(x0, x1);
+                cv = c2;
+            }
+        }
+        cy[i] = cv;
+    }
+}
+
+// lds_all: row-segment layout; cin[i]: carry into the segment of row tid + NT i.  Adds the
+// carry with a balanced re-split along the segment (the last element keeps what is left), returns
+// to the column layout, weights, forward H-point transform, w_M twiddles, stores.
+template <bool IS_GL, int W, int H, class TP>
+LL_DEV void col_forward(typename Col<IS_GL, W, H>::T *lds_all, TP tw,
+                        const typename Col<IS_GL, W, H>::WT (&wf)[nwt(IS_GL)],
+                        const typename Col<IS_GL, W, H>::T (&tm)[kR], const LargeArgs &A, int blk, int c, int t,
+                        int tid, const typename Col<IS_GL, W, H>::CT (&cin)[kNRow]) {
+    typedef Col<IS_GL, W, H> C;
+    typedef typename C::F F;
+    typedef typename C::T T;
+    typedef typename C::CT CT;
+    typedef typename C::S S;
+    const DigitWalk DW = walk_of(A);
+    const NDiv D = ndiv_of(A);
+#pragma unroll
+    for (int i = 0; i < C::NROW; ++i) {
+        const int h = tid + C::NT * i;
+        T *row = lds_all + h * kCW;
+        const u64 j0 = (u64)blk * kCW + (u64)W * h;
+        u32 r = rmod(IS_GL ? j0 : 2 * j0, A.p, D);
+        CT cv = cin[i];
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             if constexpr (IS_GL) {
                 const int b = DW.bits(r);
                 r = DW.next(r);
-                i64 v = (i64)row[cc] + cin;
+                i64 v = (i64)row[cc] + cv;
                 if (cc == kCW - 1) { row[cc] = (u64)v; break; }
-                i64 cc2;
-                v = split_i(v, b, cc2);
+                i64 c2;
+                v = split_i(v, b, c2);
                 row[cc] = (u64)v;
-                cin = cc2;
+                cv = c2;
             } else {
                 const int b0 = DW.bits(r);
                 r = DW.next(r);
                 const int b1 = DW.bits(r);
                 r = DW.next(r);
                 double c1, c2;
-                const double x0 = split_d(row[cc].re + cin, b0, c1);
+                const double x0 = split_d(row[cc].re + cv, b0, c1);
                 double x1 = row[cc].im + c1;
                 if (cc == kCW - 1) { row[cc] = cmk(x0, x1); break; }
                 x1 = split_d(x1, b1, c2);
                 row[cc] = cmk(x0, x1);
-                cin = c2;
+                cv = c2;
             }
         }
     }
     __syncthreads();
+    T x[kR];
 #pragma unroll
-    for (int s = 0; s < kR; ++s) x[s] = lds_all[(t + GH * s) * kCW + c];
-    // ---- weights (digit residues walked down the column: step = W GH elements)
-    {
-        const int tt = ll_opaque(t);
-        const u64 j0 = (u64)w + (u64)W * tt;
-        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
-        const u32 step = rmod((IS_GL ? 1ull : 2ull) * W * GH, p, log_n);
-#pragma unroll
-        for (int s = 0; s < kR; ++s) {
-            if constexpr (IS_GL) {
-                x[s] = gl_mul(gl_from_i64((i64)x[s]), tw2<u64>(A.wfL, A.wfH, DW.wexp(r), A.log_sw));
-            } else {
-                const u32 r1 = DW.next(r);
-                x[s] = cmk(x[s].re * wt2d(A.wfL, A.wfH, DW.wexp(r), A.log_sw),
-                           x[s].im * wt2d(A.wfL, A.wfH, DW.wexp(r1), A.log_sw));
-            }
-            r = DW.add(r, step);
-        }
+    for (int s = 0; s < kR; ++s) {
+        const T v = lds_all[(t + C::GH * s) * kCW + c];
+        if constexpr (IS_GL) x[s] = gl_mul(gl_from_i64((i64)v), wf[s]);
+        else x[s] = cmk(v.re * wf[2 * s], v.im * wf[2 * s + 1]);
     }
+    T *lds = lds_all + c * C::REG;
     if constexpr (S::MIXED)
-        mixed_transform<F, H, GH, kR, false, S::R1, S::R2, S::R3, S::R4>(x, lds, t, as_global<T>(A.twH));
+        mixed_transform<F, H, C::GH, kR, false, S::R1, S::R2, S::R3, S::R4>(x, lds, t, tw);
     else
-        transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, as_global<T>(A.twH));
-    // ---- twiddle w_M^{w k_h}, k_h = t + GH q, and store
-    {
-        const int tt = ll_opaque(t);
+        transform<F, H, C::GH, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, tw);
+    T *elem = reinterpret_cast<T *>(A.elem);
+    const int w = blk * kCW + c;
 #pragma unroll
-        for (int q = 0; q < kR; ++q) {
-            const u32 kh = tt + GH * q;
-            const u32 e = (u32)w * kh;
-            if constexpr (IS_GL) x[q] = gl_mul(x[q], tw2<u64>(A.tmL, A.tmH, e, A.log_s));
-            else x[q] = cmul(x[q], tw2<C64>(A.tmL, A.tmH, e, A.log_s));
-            elem[w + W * kh] = x[q];
-        }
+    for (int q = 0; q < kR; ++q) {
+        if constexpr (IS_GL) x[q] = gl_mul(x[q], tm[q]);
+        else x[q] = cmul(x[q], tm[q]);
+        elem[w + W * (t + C::GH * q)] = x[q];
     }
 }
 
-template <class F, int W, int GW, bool INV>
-LL_DEV void row_transform(typename F::T (&x)[kR], typename F::T *lds, int t, gptr<typename F::W> tw) {
+// load the column block with the inverse w_M twiddles applied (FP64: conjugate of the forward)
+template <bool IS_GL, int W, int H>
+LL_DEV void col_load_inv(typename Col<IS_GL, W, H>::T (&x)[kR], const typename Col<IS_GL, W, H>::T (&tmi)[kR],
+                         const LargeArgs &A, int w, int t) {
+    typedef Col<IS_GL, W, H> C;
+    const typename C::T *elem = reinterpret_cast<const typename C::T *>(A.elem);
+#pragma unroll
+    for (int s = 0; s < kR; ++s) {
+        x[s] = elem[w + W * (t + C::GH * s)];
+        if constexpr (IS_GL) x[s] = gl_mul(x[s], tmi[s]);
+        else x[s] = cmulc(x[s], tmi[s]);
+    }
+}
+
+LL_DEV void err_publish(unsigned long long *s_err, double eloc, const LargeArgs &A, int tid) {
+    atomicMax(s_err, (unsigned long long)__double_as_longlong(eloc));
+    __syncthreads();
+    if (tid == 0) atomicMax(A.err + A.err_slot, *s_err);
+}
+
+// ---- K1: forward column pass (first squaring of a block: carry-in from the carry buffer) --------
+template <bool IS_GL, int W, int H>
+__global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
+    typedef Col<IS_GL, W, H> C;
+    typedef typename C::T T;
+    typedef typename C::CT CT;
+    typedef typename C::WT WT;
+    __shared__ __attribute__((aligned(16))) T lds_all[kCW * C::REG];
+    __shared__ __attribute__((aligned(16))) T s_tw[H];
+    const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
+    const int blk = blockIdx.x, w = blk * kCW + c;
+    const T *elem = reinterpret_cast<const T *>(A.elem);
+    const CT *carry = reinterpret_cast<const CT *>(A.carry);
+    T x[kR];
+#pragma unroll
+    for (int s = 0; s < kR; ++s) x[s] = elem[w + W * (t + C::GH * s)];
+    CT cin[C::NROW];
+#pragma unroll
+    for (int i = 0; i < C::NROW; ++i) {
+        const int h = tid + C::NT * i;
+        cin[i] = blk > 0 ? carry[(blk - 1) * H + h] : carry[(C::NBLK - 1) * H + (h > 0 ? h - 1 : H - 1)];
+    }
+    WT wf[C::NWT];
+    col_weights<IS_GL, W, H>(wf, A.wfL, A.wfH, A, w, t);
+    T tm[kR];
+    col_twiddles<IS_GL, W, H>(tm, A.tmL, A.tmH, A.log_s, w, t);
+    stage_table<T, H, C::NT>(s_tw, A.twH, tid);
+#pragma unroll
+    for (int s = 0; s < kR; ++s) lds_all[(t + C::GH * s) * kCW + c] = x[s];
+    __syncthreads();
+    col_forward<IS_GL, W, H>(lds_all, (const T *)s_tw, wf, tm, A, blk, c, t, tid, cin);
+}
+
+// ---- K3: inverse column pass (last squaring of a block: carry-outs to the carry buffer) ---------
+template <bool IS_GL, int W, int H>
+__global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
+    typedef Col<IS_GL, W, H> C;
+    typedef typename C::T T;
+    typedef typename C::CT CT;
+    typedef typename C::WT WT;
+    __shared__ __attribute__((aligned(16))) T lds_all[kCW * C::REG];
+    __shared__ __attribute__((aligned(16))) T s_tw[H];
+    __shared__ unsigned long long s_err;
+    const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
+    const int blk = blockIdx.x, w = blk * kCW + c;
+    if (tid == 0) s_err = 0;
+    T tmi[kR];
+    col_twiddles<IS_GL, W, H>(tmi, IS_GL ? A.tmLi : A.tmL, IS_GL ? A.tmHi : A.tmH, A.log_s, w, t);
+    T x[kR];
+    col_load_inv<IS_GL, W, H>(x, tmi, A, w, t);
+    WT wi[C::NWT];
+    col_weights<IS_GL, W, H>(wi, A.wiL, A.wiH, A, w, t);
+    stage_table<T, H, C::NT>(s_tw, IS_GL ? A.twHi : A.twH, tid);
+    CT cy[C::NROW];
+    double eloc = 0.0;
+    col_inverse<IS_GL, W, H>(x, lds_all, (const T *)s_tw, wi, A, blk, c, t, tid, cy, eloc);
+    T *elem = reinterpret_cast<T *>(A.elem);
+    CT *carry = reinterpret_cast<CT *>(A.carry);
+#pragma unroll
+    for (int i = 0; i < C::NROW; ++i) {
+        const int h = tid + C::NT * i;
+        carry[blk * H + h] = cy[i];
+        const T *row = lds_all + h * kCW;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) elem[blk * kCW + cc + (size_t)W * h] = row[cc];
+    }
+    if constexpr (!IS_GL) err_publish(&s_err, eloc, A, tid);
+}
+
+// ---- K3+K1 fused: inverse column pass of one squaring and forward column pass of the next --------
+// Workgroup b runs the inverse pass of block b, publishes its row-segment carry-outs, waits for
+// block b - 1's (block 0: block NBLK - 1's, one row down, the top one wrapping to digit 0), adds
+// them and runs the forward pass.  Every workgroup publishes before it waits, and b >= 1 waits on
+// b - 1, which the in-order dispatch of workgroups has already placed; block 0 waits on the last
+// block, which needs only one free CU once the others finish.  Flags hold launch sequence
+// numbers: every workgroup adds one to a counter that only grows, so ticket / NBLK + 1 is the
+// same for all workgroups of one launch and larger than any earlier launch's.
+template <bool IS_GL, int W, int H>
+__global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fused(LargeArgs A) {
+    typedef Col<IS_GL, W, H> C;
+    typedef typename C::T T;
+    typedef typename C::CT CT;
+    typedef typename C::WT WT;
+    constexpr int NBLK = C::NBLK;
+    __shared__ __attribute__((aligned(16))) T lds_all[kCW * C::REG];
+    __shared__ __attribute__((aligned(16))) T s_tw[H];
+    __shared__ __attribute__((aligned(16))) T s_twi[IS_GL ? H : 1];
+    __shared__ unsigned long long s_err;
+    __shared__ u32 s_seq;
+    const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
+    const int blk = blockIdx.x, w = blk * kCW + c;
+    if (tid == 0) {
+        s_seq = (u32)(atomicAdd(A.ticket, 1ull) / (unsigned long long)NBLK) + 1u;
+        s_err = 0;
+    }
+    stage_table<T, H, C::NT>(s_tw, A.twH, tid);
+    if constexpr (IS_GL) stage_table<T, H, C::NT>(s_twi, A.twHi, tid);
+    T tm[kR];
+    col_twiddles<IS_GL, W, H>(tm, A.tmL, A.tmH, A.log_s, w, t);
+    T x[kR];
+    if constexpr (IS_GL) {
+        T tmi[kR];
+        col_twiddles<IS_GL, W, H>(tmi, A.tmLi, A.tmHi, A.log_s, w, t);
+        col_load_inv<IS_GL, W, H>(x, tmi, A, w, t);
+    } else {
+        col_load_inv<IS_GL, W, H>(x, tm, A, w, t);
+    }
+    WT wi[C::NWT], wf[C::NWT];
+    col_weights<IS_GL, W, H>(wi, A.wiL, A.wiH, A, w, t);
+    col_weights<IS_GL, W, H>(wf, A.wfL, A.wfH, A, w, t);
+    CT cy[C::NROW];
+    double eloc = 0.0;
+    col_inverse<IS_GL, W, H>(x, lds_all, (const T *)(IS_GL ? s_twi : s_tw), wi, A, blk, c, t, tid, cy, eloc);
+    const u32 seq = s_seq;   // written before the barriers inside col_inverse
+    CT *carry = reinterpret_cast<CT *>(A.carry);
+#pragma unroll
+    for (int i = 0; i < C::NROW; ++i) st_coh(carry + blk * H + tid + C::NT * i, cy[i]);
+    publish_flag(A.flags + blk, seq, tid);
+    if constexpr (!IS_GL) err_publish(&s_err, eloc, A, tid);
+    const int prev = blk > 0 ? blk - 1 : NBLK - 1;
+    wait_flag(A.flags + prev, seq, A.fault, tid);
+    CT cin[C::NROW];
+#pragma unroll
+    for (int i = 0; i < C::NROW; ++i) {
+        const int h = tid + C::NT * i;
+        const int src = blk > 0 ? (blk - 1) * H + h : (NBLK - 1) * H + (h > 0 ? h - 1 : H - 1);
+        ld_coh(carry + src, cin[i]);
+    }
+    col_forward<IS_GL, W, H>(lds_all, (const T *)s_tw, wf, tm, A, blk, c, t, tid, cin);
+}
+
+template <class F, int W, int GW, bool INV, class TP>
+LL_DEV void row_transform(typename F::T (&x)[kR], typename F::T *lds, int t, TP tw) {
     typedef RowSched<W> RS;
     if constexpr (RS::HYBRID) {
         hybrid_transform<F, W, GW, kR, INV, RS::M1, RS::M2, RS::M3>(x, lds, t, tw);
@@ -310,10 +595,11 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
     typedef ElemOf<IS_GL> E;
     typedef typename E::F F;
     typedef typename E::T T;
-    typedef RowSched<W> RS;
-    constexpr int GW = W / kR;
+    constexpr int GW = W / kR, NT = 2 * GW;
     constexpr int REG = W + W / 16 + 4;
     __shared__ __attribute__((aligned(16))) T lds_all[2 * REG];
+    __shared__ __attribute__((aligned(16))) T s_tw[W];
+    __shared__ __attribute__((aligned(16))) T s_twi[IS_GL ? W : 1];
     const int tid = threadIdx.x, g = tid / GW, t = tid % GW;
     const int i = blockIdx.x;
     const int row = g == 0 ? i : (i == 0 ? H / 2 : H - i);
@@ -323,11 +609,18 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
     T x[kR];
 #pragma unroll
     for (int s = 0; s < kR; ++s) x[s] = elem[t + GW * s];
-    row_transform<F, W, GW, false>(x, lds, t, as_global<T>(A.twW));
+    C64 wk[IS_GL ? 1 : kR];
+    if constexpr (!IS_GL) {
+#pragma unroll
+        for (int q = 0; q < kR; ++q) wk[q] = tw2<C64>(A.tmL, A.tmH, (u32)row + (u32)H * (u32)(t + GW * q), A.log_s);
+    }
+    stage_table<T, W, NT>(s_tw, A.twW, tid);
+    if constexpr (IS_GL) stage_table<T, W, NT>(s_twi, A.twWi, tid);
+    row_transform<F, W, GW, false>(x, lds, t, (const T *)s_tw);
     if constexpr (IS_GL) {
 #pragma unroll
         for (int q = 0; q < kR; ++q) x[q] = gl_sqr(x[q]);
-        row_transform<F, W, GW, true>(x, lds, t, as_global<T>(A.twWi));
+        row_transform<F, W, GW, true>(x, lds, t, (const T *)s_twi);
     } else {
         const int tt = ll_opaque(t);
         __syncthreads();
@@ -343,123 +636,14 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
             const int kwp = row == 0 ? (kw == 0 ? 0 : W - kw) : (W - 1 - kw);
             const C64 zm = plds[lds_pad(kwp)];
             const C64 D = csub(x[q], cconj(zm));
-            const u32 k = (u32)row + (u32)H * (u32)kw;
-            const C64 wk = tw2<C64>(A.tmL, A.tmH, k, A.log_s);
-            const C64 AD = cmul(cmk(1.0 + wk.re, wk.im), csqr(D));
+            const C64 AD = cmul(cmk(1.0 + wk[q].re, wk[q].im), csqr(D));
             const C64 Z2 = csqr(x[q]);
             x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
         }
-        row_transform<F, W, GW, true>(x, lds, t, as_global<T>(A.twW));
+        row_transform<F, W, GW, true>(x, lds, t, (const T *)s_tw);
     }
 #pragma unroll
     for (int s = 0; s < kR; ++s) elem[t + GW * s] = x[s];
-}
-
-// ---- K3 -----------------------------------------------------------------------------------
-template <bool IS_GL, int W, int H>
-__global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
-    typedef ElemOf<IS_GL> E;
-    typedef typename E::F F;
-    typedef typename E::T T;
-    typedef typename E::CT CT;
-    typedef ColSched<H> S;
-    constexpr int GH = H / kR, NT = kCW * GH;
-    constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
-    __shared__ __attribute__((aligned(16))) T lds_all[kCW * REG];
-    __shared__ unsigned long long s_err;
-    const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
-    const int blk = blockIdx.x, w = blk * kCW + c;
-    T *lds = lds_all + c * REG;
-    T *elem = reinterpret_cast<T *>(A.elem);
-    CT *carry = reinterpret_cast<CT *>(A.carry);
-    const u32 p = A.p;
-    const NDiv log_n = ndiv_of(A);
-    if (tid == 0) s_err = 0;
-
-    T x[kR];
-    {
-        const int tt = ll_opaque(t);
-#pragma unroll
-        for (int s = 0; s < kR; ++s) {
-            const u32 kh = tt + GH * s;
-            const u32 e = (u32)w * kh;
-            x[s] = elem[w + W * kh];
-            if constexpr (IS_GL) x[s] = gl_mul(x[s], tw2<u64>(A.tmLi, A.tmHi, e, A.log_s));
-            else x[s] = cmulc(x[s], tw2<C64>(A.tmL, A.tmH, e, A.log_s));
-        }
-    }
-    if constexpr (S::MIXED)
-        mixed_transform<F, H, GH, kR, true, S::R1, S::R2, S::R3, S::R4>(x, lds, t, as_global<T>(A.twH));
-    else
-        transform<F, H, GH, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, as_global<T>(IS_GL ? A.twHi : A.twH));
-    // ---- unweight and round (rows h = t + GH q)
-    double eloc = 0.0;
-    const DigitWalk DW = walk_of(A);
-    {
-        const int tt = ll_opaque(t);
-        const u64 j0 = (u64)w + (u64)W * tt;
-        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
-        const u32 step = rmod((IS_GL ? 1ull : 2ull) * W * GH, p, log_n);
-#pragma unroll
-        for (int q = 0; q < kR; ++q) {
-            const u64 j = j0 + (u64)W * GH * q;
-            const u32 rq = r;
-            r = DW.add(r, step);
-            if constexpr (IS_GL) {
-                i64 v = gl_to_i64(gl_mul(x[q], tw2<u64>(A.wiL, A.wiH, DW.wexp(rq), A.log_sw)));
-                if (j == 0) v -= 2;
-                x[q] = (u64)v;
-            } else {
-                const double v0 = x[q].re * wt2d(A.wiL, A.wiH, DW.wexp(rq), A.log_sw);
-                const double v1 = x[q].im * wt2d(A.wiL, A.wiH, DW.wexp(DW.next(rq)), A.log_sw);
-                double r0 = __builtin_rint(v0);
-                const double r1 = __builtin_rint(v1);
-                eloc = fmax(eloc, fmax(fabs(v0 - r0), fabs(v1 - r1)));
-                if (j == 0) r0 -= 2.0;
-                x[q] = cmk(r0, r1);
-            }
-        }
-    }
-    // ---- balanced carry along each row segment of kCW elements
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kR; ++q) lds_all[(t + GH * q) * kCW + c] = x[q];
-    __syncthreads();
-    for (int h = tid; h < H; h += NT) {
-        T *row = lds_all + h * kCW;
-        const u64 j0 = (u64)blk * kCW + (u64)W * h;
-        u32 r = rmod(IS_GL ? j0 : 2 * j0, p, log_n);
-        CT cy = 0;
-#pragma unroll
-        for (int cc = 0; cc < kCW; ++cc) {
-            if constexpr (IS_GL) {
-                i64 c2;
-                const i64 v = split_i((i64)row[cc] + cy, DW.bits(r), c2);
-                r = DW.next(r);
-                row[cc] = (u64)v;
-                cy = c2;
-            } else {
-                double c1, c2;
-                const int b0 = DW.bits(r);
-                r = DW.next(r);
-                const int b1 = DW.bits(r);
-                r = DW.next(r);
-                const double x0 = split_d(row[cc].re + cy, b0, c1);
-                const double x1 = split_d(row[cc].im + c1, b1, c2);
-                row[cc] = cmk(x0, x1);
-                cy = c2;
-            }
-        }
-        carry[blk * H + h] = cy;
-        // the segment is contiguous in memory
-#pragma unroll
-        for (int cc = 0; cc < kCW; ++cc) elem[blk * kCW + cc + (size_t)W * h] = row[cc];
-    }
-    if constexpr (!IS_GL) {
-        atomicMax(&s_err, (unsigned long long)__double_as_longlong(eloc));
-        __syncthreads();
-        if (tid == 0) atomicMax(A.err + A.err_slot, s_err);
-    }
 }
 
 // ---- carry-in only (after the last squaring, before normalisation) ----------------------------
@@ -593,18 +777,10 @@ __global__ void __launch_bounds__(kNormThreads) lg_normalize(const void *elem_v,
 
 namespace llh {
 
-struct LargeLaunch {
-    void (*fwd)(LargeArgs);
-    void (*rows)(LargeArgs);
-    void (*inv)(LargeArgs);
-    void (*apply)(LargeArgs);
-    int w_len, h_len;   // W and H (either may carry the odd factor)
-};
-
 template <bool IS_GL, int W, int H>
 static LargeLaunch make_launch() {
     return LargeLaunch{lg_cols_fwd<IS_GL, W, H>, lg_rows<IS_GL, W, H>, lg_cols_inv<IS_GL, W, H>,
-                       lg_apply_carry<IS_GL, W, H>, W, H};
+                       lg_cols_fused<IS_GL, W, H>, lg_apply_carry<IS_GL, W, H>, W, H};
 }
 
 // element count M -> (W, H) split and kernels.  FP64: M = m 2^k with m in {1, 3, 5, 7, 15},
@@ -658,13 +834,22 @@ bool large_launch_for(int transform, u64 M, LargeLaunch &L) {
 #undef LL_ODD
 }
 
-hipError_t large_step(const LargeLaunch &L, const LargeArgs &A, hipStream_t st) {
+int large_blocks(const LargeLaunch &L) { return L.w_len / kCW; }
+
+// k squarings: forward column pass, then per squaring the row pass and the fused column pass
+// (the last squaring ends with the inverse column pass, leaving its carries in the carry buffer).
+// Squaring i reports its round-off in err slot i.
+hipError_t large_squarings(const LargeLaunch &L, LargeArgs A, int k, hipStream_t st) {
     const int W = L.w_len, H = L.h_len;
     const dim3 gcol(W / kCW), bcol(kCW * (H / kR));
     const dim3 grow(H / 2), brow(2 * (W / kR));
     hipLaunchKernelGGL(L.fwd, gcol, bcol, 0, st, A);
-    hipLaunchKernelGGL(L.rows, grow, brow, 0, st, A);
-    hipLaunchKernelGGL(L.inv, gcol, bcol, 0, st, A);
+    for (int i = 0; i < k; ++i) {
+        A.err_slot = (u32)i;
+        hipLaunchKernelGGL(L.rows, grow, brow, 0, st, A);
+        if (i + 1 < k) hipLaunchKernelGGL(L.fused, gcol, bcol, 0, st, A);
+        else hipLaunchKernelGGL(L.inv, gcol, bcol, 0, st, A);
+    }
     return hipGetLastError();
 }
 
