@@ -28,6 +28,7 @@
 #include "ll_dft.hpp"
 #include "ll_internal.hpp"
 #include <type_traits>
+#include <vector>
 
 namespace {
 
@@ -48,6 +49,21 @@ struct Ctl {
 constexpr int kSmemBytes = kWG * 8 * 16 * 17 / 16;
 __shared__ __attribute__((aligned(16))) char g_smem[kSmemBytes];
 __shared__ Ctl g_ctl;
+// FP64 two-level weight tables of the class being run (ClassDesc::wtab)
+__shared__ double g_wtab[4 * llh::kWtabSlot];
+
+// Radix schedule of each size class (index log2 M): M, G = M / R threads per exponent, R
+// elements per thread, radices R1 .. R4 (first = last = R; 0 = no pass).
+struct Sch {
+    int M, G, R, R1, R2, R3, R4;
+};
+constexpr Sch kSch[13] = {
+    {0, 0, 0, 0, 0, 0, 0},          {0, 0, 0, 0, 0, 0, 0},       {0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0},          {16, 4, 4, 4, 4, 0, 0},      {32, 8, 4, 4, 2, 4, 0},
+    {64, 8, 8, 8, 8, 0, 0},         {128, 16, 8, 8, 2, 8, 0},    {256, 32, 8, 8, 4, 8, 0},
+    {512, 64, 8, 8, 8, 8, 0},       {1024, 128, 8, 8, 2, 8, 8},  {2048, 256, 8, 8, 4, 8, 8},
+    {4096, 512, 8, 8, 8, 8, 8},
+};
 
 // Exact canonical residue of N near-balanced digits (natural order, i64, in LDS):
 // full carry propagation with wrap (2^p == 1), then Res64 = low 64 bits of the value in
@@ -104,9 +120,20 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     const int log_n = cd.log_n;
     const u32 N = 1u << log_n, nmask = N - 1;
     const gptr<W> tw = as_global<W>(cd.tw);
-    const gptr<W> twi = IS_GL ? as_global<W>(cd.twi) : tw;
+    const gptr<W> twp = as_global<W>(cd.twp);
+    const gptr<W> twpi = IS_GL ? as_global<W>(cd.twpi) : twp;
     const gptr<WT> wf = as_global<WT>(cd.wf);
     const gptr<WT> wi = as_global<WT>(cd.wi);
+    const u32 smask = (1u << cd.log_s) - 1;
+    const int log_s = cd.log_s;
+    if constexpr (!IS_GL) {
+        static_assert(4 * llh::kWtabSlot <= kWG, "one weight entry per thread");
+        if (tid < 4 * llh::kWtabSlot) g_wtab[tid] = ld_table(as_global<double>(cd.wtab), (u32)tid);
+    }
+    // weight of exponent e (forward: slots 0, 1; inverse: slots 2, 3)
+    auto wlds = [&](u32 e, int inv) {
+        return g_wtab[(2 * inv) * llh::kWtabSlot + (e & smask)] * g_wtab[(2 * inv + 1) * llh::kWtabSlot + (e >> log_s)];
+    };
 
     u32 item = kNone, p = N + 1;
     u64 target = 0, it = 0, t0 = 0;
@@ -157,9 +184,9 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const u32 j = tw0 + G * s;
                 const u32 dp0 = 2u * j * pw;
                 const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - pw) & nmask;
-                x[s] = cmk(x[s].re * ld_table(wf, e0), x[s].im * ld_table(wf, e1));
+                x[s] = cmk(x[s].re * wlds(e0, 0), x[s].im * wlds(e1, 0));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, false>(x, lds, t, tw);
+            transform<F, M, G, R, R1, R2, R3, R4, false, true>(x, lds, t, twp);
             // ---- square the real spectrum in packed form (pairs k, M-k)
             const int tq = ll_opaque(t);
             __syncthreads();
@@ -178,7 +205,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const C64 Z2 = csqr(x[q]);
                 x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, true>(x, lds, t, tw);
+            transform<F, M, G, R, R1, R2, R3, R4, true, true>(x, lds, t, twp);
             // ---- unweight, round, error, -2, carry within the pair
             double cout[R];
             const int tu = ll_opaque(t);
@@ -191,7 +218,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                           c2 = (dp0 + 2u * pu + N - 1) >> log_n;
                 const int b0 = (int)(c1 - c0), b1 = (int)(c2 - c1);
                 const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - pu) & nmask;
-                const double v0 = x[s].re * ld_table(wi, e0), v1 = x[s].im * ld_table(wi, e1);
+                const double v0 = x[s].re * wlds(e0, 1), v1 = x[s].im * wlds(e1, 1);
                 double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
                 eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
                 if (j == 0 && has) r0 -= 2.0;
@@ -251,10 +278,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const u32 e = (0u - j * pw) & nmask;
                 x[s] = gl_mul(gl_from_i64((i64)x[s]), ld_table(wf, e));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, false>(x, lds, t, tw);
+            transform<F, M, G, R, R1, R2, R3, R4, false, true>(x, lds, t, twp);
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = gl_sqr(x[q]);
-            transform<F, M, G, R, R1, R2, R3, R4, true>(x, lds, t, twi);
+            transform<F, M, G, R, R1, R2, R3, R4, true, true>(x, lds, t, twpi);
             i64 cout[R];
             const int tu = ll_opaque(t);
             const u32 pu = ll_opaque(p);
@@ -397,15 +424,14 @@ ll_batch_kernel(const llh::ClassDesc *__restrict__ classes, int ncls,
     for (int c = 0; c < ncls; ++c) {
         const llh::ClassDesc cd = classes[c];
         switch (cd.log_m) {
-            case 12: run_class<IS_GL, 4096, 512, 8, 8, 8, 8, 8>(cd, items, out, counters + c); break;
-            case 11: run_class<IS_GL, 2048, 256, 8, 8, 4, 8, 8>(cd, items, out, counters + c); break;
-            case 10: run_class<IS_GL, 1024, 128, 8, 8, 2, 8, 8>(cd, items, out, counters + c); break;
-            case 9: run_class<IS_GL, 512, 64, 8, 8, 8, 8, 0>(cd, items, out, counters + c); break;
-            case 8: run_class<IS_GL, 256, 32, 8, 8, 4, 8, 0>(cd, items, out, counters + c); break;
-            case 7: run_class<IS_GL, 128, 16, 8, 8, 2, 8, 0>(cd, items, out, counters + c); break;
-            case 6: run_class<IS_GL, 64, 8, 8, 8, 8, 0, 0>(cd, items, out, counters + c); break;
-            case 5: run_class<IS_GL, 32, 8, 4, 4, 2, 4, 0>(cd, items, out, counters + c); break;
-            case 4: run_class<IS_GL, 16, 4, 4, 4, 4, 0, 0>(cd, items, out, counters + c); break;
+#define LL_CLASS(L)                                                                                       \
+    case L:                                                                                               \
+        run_class<IS_GL, kSch[L].M, kSch[L].G, kSch[L].R, kSch[L].R1, kSch[L].R2, kSch[L].R3, kSch[L].R4>( \
+            cd, items, out, counters + c);                                                                \
+        break;
+            LL_CLASS(12) LL_CLASS(11) LL_CLASS(10) LL_CLASS(9) LL_CLASS(8) LL_CLASS(7) LL_CLASS(6)
+            LL_CLASS(5) LL_CLASS(4)
+#undef LL_CLASS
             case 0: run_tiny(cd, items, out, counters + c); break;
             default: break;
         }
@@ -427,6 +453,30 @@ hipError_t launch_batch(int transform, const ClassDesc *d_classes, int ncls, con
         hipLaunchKernelGGL(ll_batch_kernel<false>, dim3(grid), dim3(kWG), 0, stream, d_classes, ncls,
                            d_items, d_out, d_counters);
     return hipGetLastError();
+}
+
+template <int L>
+static void pass_exponents(std::vector<u32> &ex) {
+    typedef PassTable<kSch[L].R1, kSch[L].R2, kSch[L].R3, kSch[L].R4, 0> PT;
+    static_assert(PT::M == kSch[L].M, "schedule must cover the class length");
+    ex.resize(PT::SIZE);
+    for (int e = 0; e < PT::SIZE; ++e) ex[e] = PT::exponent(e);
+}
+
+// Full-table exponent of every entry of the class's per-pass twiddle table.
+bool batch_pass_exponents(int log_m, std::vector<u32> &ex) {
+    switch (log_m) {
+        case 4: pass_exponents<4>(ex); return true;
+        case 5: pass_exponents<5>(ex); return true;
+        case 6: pass_exponents<6>(ex); return true;
+        case 7: pass_exponents<7>(ex); return true;
+        case 8: pass_exponents<8>(ex); return true;
+        case 9: pass_exponents<9>(ex); return true;
+        case 10: pass_exponents<10>(ex); return true;
+        case 11: pass_exponents<11>(ex); return true;
+        case 12: pass_exponents<12>(ex); return true;
+        default: return false;
+    }
 }
 
 int batch_blocks_per_cu(int transform) {

@@ -45,6 +45,8 @@ void set_error(const std::string &m) { g_err = m; }
 
 struct TableSet {
     void *tw = nullptr, *twi = nullptr, *wf = nullptr, *wi = nullptr;
+    void *twp = nullptr, *twpi = nullptr, *wtab = nullptr;
+    int log_s = 0;
 };
 
 struct DeviceState {
@@ -172,12 +174,44 @@ static int get_tables(int transform, int log_n, TableSet &out) {
             wf[e] = (double)w;
             wi[e] = (double)(1.0L / (w * (long double)M));
         }
+        // per-pass twiddle order of the class's schedule
+        std::vector<u32> ex;
+        if (!batch_pass_exponents(log_n - 1, ex)) {
+            set_error("no batch schedule for this length");
+            return LLH_ERR_UNSUPPORTED;
+        }
+        std::vector<double> twp(2 * ex.size() + 2);
+        for (size_t e = 0; e < ex.size(); ++e) {
+            twp[2 * e] = tw[2 * ex[e]];
+            twp[2 * e + 1] = tw[2 * ex[e] + 1];
+        }
+        // two-level weights: e = lo + S hi, w(e) = 2^(lo/N) 2^(S hi/N) (inverse with 1/M)
+        const int log_s = (log_n + 1) / 2;
+        const u64 S = (u64)1 << log_s, HS = N >> log_s;
+        if (S > (u64)kWtabSlot || HS > (u64)kWtabSlot) {
+            set_error("weight table slot too small");
+            return LLH_ERR_UNSUPPORTED;
+        }
+        std::vector<double> wtab(4 * kWtabSlot, 0.0);
+        for (u64 i = 0; i < S; ++i) {
+            wtab[i] = (double)exp2l((long double)i / (long double)N);
+            wtab[2 * kWtabSlot + i] = (double)exp2l(-(long double)i / (long double)N);
+        }
+        for (u64 i = 0; i < HS; ++i) {
+            wtab[kWtabSlot + i] = (double)exp2l((long double)(i * S) / (long double)N);
+            wtab[3 * kWtabSlot + i] = (double)(exp2l(-(long double)(i * S) / (long double)N) / (long double)M);
+        }
+        ts.log_s = log_s;
         HIPCHK(hipMalloc(&ts.tw, tw.size() * 8));
         HIPCHK(hipMalloc(&ts.wf, N * 8));
         HIPCHK(hipMalloc(&ts.wi, N * 8));
+        HIPCHK(hipMalloc(&ts.twp, twp.size() * 8));
+        HIPCHK(hipMalloc(&ts.wtab, wtab.size() * 8));
         HIPCHK(hipMemcpy(ts.tw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ts.wf, wf.data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ts.wi, wi.data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.twp, twp.data(), twp.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.wtab, wtab.data(), wtab.size() * 8, hipMemcpyHostToDevice));
     } else {
         const u64 w = gl_root_unity_aligned(N), wv = gl_inv_h(w);
         const u64 r2 = gl_root2_h(N), r2i = gl_inv_h(r2), ninv = gl_inv_h(N);
@@ -191,14 +225,28 @@ static int get_tables(int transform, int log_n, TableSet &out) {
             tw[j] = a; twi[j] = b; wf[j] = c; wi[j] = d;
             a = gl_mul_h(a, w); b = gl_mul_h(b, wv); c = gl_mul_h(c, r2); d = gl_mul_h(d, r2i);
         }
+        std::vector<u32> ex;
+        if (!batch_pass_exponents(log_n, ex)) {
+            set_error("no batch schedule for this length");
+            return LLH_ERR_UNSUPPORTED;
+        }
+        std::vector<u64> twp(ex.size() + 1), twpi(ex.size() + 1);
+        for (size_t e = 0; e < ex.size(); ++e) {
+            twp[e] = tw[ex[e]];
+            twpi[e] = twi[ex[e]];
+        }
         HIPCHK(hipMalloc(&ts.tw, N * 8));
         HIPCHK(hipMalloc(&ts.twi, N * 8));
         HIPCHK(hipMalloc(&ts.wf, N * 8));
         HIPCHK(hipMalloc(&ts.wi, N * 8));
+        HIPCHK(hipMalloc(&ts.twp, twp.size() * 8));
+        HIPCHK(hipMalloc(&ts.twpi, twpi.size() * 8));
         HIPCHK(hipMemcpy(ts.tw, tw.data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ts.twi, twi.data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ts.wf, wf.data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ts.wi, wi.data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.twp, twp.data(), twp.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ts.twpi, twpi.data(), twpi.size() * 8, hipMemcpyHostToDevice));
     }
     g_dev.tables[key] = ts;
     out = ts;
@@ -282,6 +330,7 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
                 int rc = get_tables(transform, w.log_n, ts);
                 if (rc) return rc;
                 cd.tw = ts.tw; cd.twi = ts.twi; cd.wf = ts.wf; cd.wi = ts.wi;
+                cd.twp = ts.twp; cd.twpi = ts.twpi; cd.wtab = ts.wtab; cd.log_s = ts.log_s;
             }
             cls.push_back(cd);
         }

@@ -18,6 +18,48 @@
 #include "ll_common.hpp"
 
 LL_HD constexpr int lds_pad(int i) { return i + (i >> 4); }
+// One pad element per 2^PS: the stride-2^PS stores of a radix-2^PS first pass land on distinct
+// banks for every 8 consecutive lanes (16-byte elements).  PS = 4 is lds_pad.
+template <int PS>
+LL_HD constexpr int lds_pad_s(int i) { return i + (i >> PS); }
+
+// Twiddles of a transform with radices R1 .. R5 (0 = no such pass) in per-pass order: pass p >= 2
+// (L = R1 ... R_{p-1}) keeps w_M^{s k M / (R_p L)}, s in [1, R_p), k in [0, L), at
+// O_p + (s - 1) L + k, so the lanes of a pass (consecutive k) read consecutive entries.
+template <int R1, int R2, int R3, int R4, int R5>
+struct PassTable {
+    static constexpr int L2 = R1;
+    static constexpr int L3 = L2 * (R2 ? R2 : 1);
+    static constexpr int L4 = L3 * (R3 ? R3 : 1);
+    static constexpr int L5 = L4 * (R4 ? R4 : 1);
+    static constexpr int O2 = 0;
+    static constexpr int O3 = O2 + (R2 ? (R2 - 1) * L2 : 0);
+    static constexpr int O4 = O3 + (R3 ? (R3 - 1) * L3 : 0);
+    static constexpr int O5 = O4 + (R4 ? (R4 - 1) * L4 : 0);
+    static constexpr int SIZE = O5 + (R5 ? (R5 - 1) * L5 : 0);
+    static constexpr int M = L5 * (R5 ? R5 : 1);
+    // full-table exponent of per-pass entry e
+    LL_HD static constexpr u32 exponent(int e) {
+        return e < O3 ? one(e - O2, R2, L2) : e < O4 ? one(e - O3, R3, L3) : e < O5 ? one(e - O4, R4, L4)
+                                                                             : one(e - O5, R5, L5);
+    }
+    LL_HD static constexpr u32 one(int d, int r, int L) {
+        return (u32)((d / L + 1) * (d % L) * (M / (r * L)));
+    }
+};
+
+// Copy the per-pass twiddle table of PassTable<R1..R5> out of the full table src (w_M^j, j < M)
+// into LDS, all NT threads of the workgroup.
+template <class W, int NT, int R1, int R2, int R3, int R4, int R5>
+LL_DEV void stage_pass_table(W *dst, const void *src, int tid) {
+    typedef PassTable<R1, R2, R3, R4, R5> PT;
+    const W LL_GLOBAL *g = (const W LL_GLOBAL *)src;
+#pragma unroll
+    for (int i = 0; i < (PT::SIZE + NT - 1) / NT; ++i) {
+        const int e = tid + NT * i;
+        if (PT::SIZE % NT == 0 || e < PT::SIZE) dst[e] = ld_table(g, PT::exponent(e));
+    }
+}
 
 // Twiddle table access: a global table (gptr) or a copy staged in LDS (generic pointer to
 // __shared__ memory, which the compiler lowers to ds_read once the kernel is inlined).
@@ -166,7 +208,10 @@ LL_DEV void dft_r(typename F::T *x) {
 // threads; t is the thread's index inside its group.  `tw` holds w_M^j, j < M (for the
 // inverse transform the C64 path conjugates, the GL path is handed the inverse table).
 // FIRST: inputs come from registers;  LAST: outputs stay in registers.
-template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV, class TP>
+// OFF >= 0: `tw` is a per-pass table (PassTable) and this pass's entries start at OFF.
+// PS: LDS pad shift (lds_pad_s).
+template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV, int OFF = -1, int PS = 4,
+          class TP>
 LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(R % r == 0, "radix must divide the per-thread count");
     static_assert(!FIRST || r == R, "first pass must use the full per-thread radix");
@@ -179,7 +224,7 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int s = 0; s < r; ++s) x[b * r + s] = lds[lds_pad(t + G * b + s * STRIDE)];
+            for (int s = 0; s < r; ++s) x[b * r + s] = lds[lds_pad_s<PS>(t + G * b + s * STRIDE)];
     }
     if constexpr (L > 1) {
 #pragma unroll
@@ -188,8 +233,10 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
             const int k = i % L;
             constexpr int TSTEP = M / (r * L);
 #pragma unroll
-            for (int s = 1; s < r; ++s)
-                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], tw_at<typename F::W>(tw, (u32)(s * k * TSTEP)));
+            for (int s = 1; s < r; ++s) {
+                const u32 e = OFF >= 0 ? (u32)(OFF + (s - 1) * L + k) : (u32)(s * k * TSTEP);
+                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], tw_at<typename F::W>(tw, e));
+            }
         }
     }
 #pragma unroll
@@ -201,29 +248,33 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
             const int i = t + G * b;
             const int k = i % L;
 #pragma unroll
-            for (int q = 0; q < r; ++q) lds[lds_pad((i - k) * r + k + q * L)] = x[b * r + q];
+            for (int q = 0; q < r; ++q) lds[lds_pad_s<PS>((i - k) * r + k + q * L)] = x[b * r + q];
         }
     }
 }
 
-// Whole transform: two, three or four passes (R3 == 0: two; R4 == 0: three).
-template <class F, int M, int G, int R, int R1, int R2, int R3, int R4, bool INV, class TP>
+// Whole transform: two, three or four passes (R3 == 0: two; R4 == 0: three).  PP: `tw` is the
+// per-pass table of PassTable<R1, R2, R3, R4, 0>; PS: LDS pad shift.
+template <class F, int M, int G, int R, int R1, int R2, int R3, int R4, bool INV, bool PP = false, int PS = 4,
+          class TP>
 LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
+    typedef PassTable<R1, R2, R3, R4, 0> PT;
+    constexpr int O2 = PP ? PT::O2 : -1, O3 = PP ? PT::O3 : -1, O4 = PP ? PT::O4 : -1;
     if constexpr (R4 != 0) {
         static_assert(R1 * R2 * R3 * R4 == M, "radices must multiply to M");
-        stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R2, R1, false, false, INV>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R3, R1 * R2, false, false, INV>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R4, R1 * R2 * R3, false, true, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R1, 1, true, false, INV, -1, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, false, INV, O2, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R3, R1 * R2, false, false, INV, O3, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R4, R1 * R2 * R3, false, true, INV, O4, PS>(x, lds, t, tw);
     } else if constexpr (R3 == 0) {
         static_assert(R1 * R2 == M, "radices must multiply to M");
-        stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R2, R1, false, true, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R1, 1, true, false, INV, -1, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, true, INV, O2, PS>(x, lds, t, tw);
     } else {
         static_assert(R1 * R2 * R3 == M, "radices must multiply to M");
-        stockham_pass<F, M, G, R, R1, 1, true, false, INV>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R2, R1, false, false, INV>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R3, R1 * R2, false, true, INV>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R1, 1, true, false, INV, -1, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, false, INV, O2, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R3, R1 * R2, false, true, INV, O3, PS>(x, lds, t, tw);
     }
 }
 
@@ -232,7 +283,7 @@ LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
 // Used where the register-blocked form does not apply (lengths with an odd factor).  The data
 // sits in LDS in natural order (padded); every pass reads its butterflies' inputs, barriers,
 // and writes the outputs in place.  A thread runs ceil((H/r)/G) butterflies per pass.
-template <class F, int H, int G, int r, int L, bool INV, class TP>
+template <class F, int H, int G, int r, int L, bool INV, int OFF = -1, int PS = 4, class TP>
 LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
     typedef typename F::T T;
     constexpr int NBF = H / r;
@@ -248,11 +299,13 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
         if (i < NBF) {
             const int k = i % L;
 #pragma unroll
-            for (int s = 0; s < r; ++s) y[b][s] = lds[lds_pad(i + s * NBF)];
+            for (int s = 0; s < r; ++s) y[b][s] = lds[lds_pad_s<PS>(i + s * NBF)];
             if constexpr (L > 1) {
 #pragma unroll
-                for (int s = 1; s < r; ++s)
-                    y[b][s] = F::template twiddle<INV>(y[b][s], tw_at<typename F::W>(tw, (u32)(s * k * TSTEP)));
+                for (int s = 1; s < r; ++s) {
+                    const u32 e = OFF >= 0 ? (u32)(OFF + (s - 1) * L + k) : (u32)(s * k * TSTEP);
+                    y[b][s] = F::template twiddle<INV>(y[b][s], tw_at<typename F::W>(tw, e));
+                }
             }
             dft_r<F, r, INV>(y[b]);
         }
@@ -264,27 +317,29 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
         if (i < NBF) {
             const int k = i % L;
 #pragma unroll
-            for (int q = 0; q < r; ++q) lds[lds_pad((i - k) * r + k + q * L)] = y[b][q];
+            for (int q = 0; q < r; ++q) lds[lds_pad_s<PS>((i - k) * r + k + q * L)] = y[b][q];
         }
     }
 }
 
 // x (column layout: thread t holds elements t + G s, s < R) -> transform -> x (same layout)
-template <class F, int H, int G, int R, bool INV, int R1, int R2, int R3, int R4, class TP>
+template <class F, int H, int G, int R, bool INV, int R1, int R2, int R3, int R4, bool PP = false, int PS = 4,
+          class TP>
 LL_DEV void mixed_transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(G * R == H, "column layout must cover the length");
     static_assert(R1 * (R2 ? R2 : 1) * (R3 ? R3 : 1) * (R4 ? R4 : 1) == H, "radices must multiply to H");
+    typedef PassTable<R1, R2, R3, R4, 0> PT;
     const int tt = ll_opaque(t);
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < R; ++s) lds[lds_pad(tt + G * s)] = x[s];
-    mixed_pass<F, H, G, R1, 1, INV>(lds, t, tw);
-    if constexpr (R2) mixed_pass<F, H, G, R2, R1, INV>(lds, t, tw);
-    if constexpr (R3) mixed_pass<F, H, G, R3, R1 * R2, INV>(lds, t, tw);
-    if constexpr (R4) mixed_pass<F, H, G, R4, R1 * R2 * R3, INV>(lds, t, tw);
+    for (int s = 0; s < R; ++s) lds[lds_pad_s<PS>(tt + G * s)] = x[s];
+    mixed_pass<F, H, G, R1, 1, INV, -1, PS>(lds, t, tw);
+    if constexpr (R2) mixed_pass<F, H, G, R2, R1, INV, PP ? PT::O2 : -1, PS>(lds, t, tw);
+    if constexpr (R3) mixed_pass<F, H, G, R3, R1 * R2, INV, PP ? PT::O3 : -1, PS>(lds, t, tw);
+    if constexpr (R4) mixed_pass<F, H, G, R4, R1 * R2 * R3, INV, PP ? PT::O4 : -1, PS>(lds, t, tw);
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < R; ++s) x[s] = lds[lds_pad(tt + G * s)];
+    for (int s = 0; s < R; ++s) x[s] = lds[lds_pad_s<PS>(tt + G * s)];
 }
 
 
@@ -292,13 +347,17 @@ LL_DEV void mixed_transform(typename F::T (&x)[R], typename F::T *lds, int t, TP
 // For a length W = 8 * m * 8 (m = product of M1 M2 M3, any of 2..16 including 3, 5, 7):
 // the first radix-8 pass reads the column layout from registers, the middle passes run
 // through LDS (mixed_pass), the last radix-8 pass leaves the column layout in registers.
-template <class F, int W, int G, int R, bool INV, int M1, int M2, int M3, class TP>
+template <class F, int W, int G, int R, bool INV, int M1, int M2, int M3, bool PP = false, int PS = 4, class TP>
 LL_DEV void hybrid_transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(R == 8 && G * R == W, "hybrid transform runs 8 elements per thread");
     static_assert(8 * M1 * (M2 ? M2 : 1) * (M3 ? M3 : 1) * 8 == W, "radices must multiply to W");
-    stockham_pass<F, W, G, R, 8, 1, true, false, INV>(x, lds, t, tw);
-    mixed_pass<F, W, G, M1, 8, INV>(lds, t, tw);
-    if constexpr (M2 != 0) mixed_pass<F, W, G, M2, 8 * M1, INV>(lds, t, tw);
-    if constexpr (M3 != 0) mixed_pass<F, W, G, M3, 8 * M1 * M2, INV>(lds, t, tw);
-    stockham_pass<F, W, G, R, 8, W / 8, false, true, INV>(x, lds, t, tw);
+    // radix list 8, M1, [M2], [M3], 8 packed without gaps for the per-pass table
+    constexpr int P3 = M2 ? M2 : 8, P4 = M2 ? (M3 ? M3 : 8) : 0, P5 = (M2 && M3) ? 8 : 0;
+    typedef PassTable<8, M1, P3, P4, P5> PT;
+    constexpr int OL = !PP ? -1 : (M2 == 0 ? PT::O3 : M3 == 0 ? PT::O4 : PT::O5);
+    stockham_pass<F, W, G, R, 8, 1, true, false, INV, -1, PS>(x, lds, t, tw);
+    mixed_pass<F, W, G, M1, 8, INV, PP ? PT::O2 : -1, PS>(lds, t, tw);
+    if constexpr (M2 != 0) mixed_pass<F, W, G, M2, 8 * M1, INV, PP ? PT::O3 : -1, PS>(lds, t, tw);
+    if constexpr (M3 != 0) mixed_pass<F, W, G, M3, 8 * M1 * M2, INV, PP ? PT::O4 : -1, PS>(lds, t, tw);
+    stockham_pass<F, W, G, R, 8, W / 8, false, true, INV, OL, PS>(x, lds, t, tw);
 }

@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <vector>
 
 namespace llh {
 
@@ -51,7 +52,17 @@ struct ClassDesc {
     const void *twi;    // GL: inverse roots; C64: unused (conjugated in registers)
     const void *wf;     // forward weights indexed by e = (-j p) mod N: 2^(e/N) / w2^e
     const void *wi;     // inverse weights incl. the transform's 1/M or 1/N scale
+    const void *twp;    // tw in per-pass order of the class's radix schedule (PassTable)
+    const void *twpi;   // GL: twi in per-pass order
+    const void *wtab;   // FP64: two-level weights, 4 x kWtabSlot doubles: forward low | forward
+                        // high | inverse low | inverse high, w(e) = low[e & (S-1)] * high[e >> log_s]
+    i32 log_s;          // S = 2^log_s
+    i32 pad;
 };
+constexpr int kWtabSlot = 128;
+
+// Batched engine: radix schedule of a class and the per-pass twiddle order (ll_batch.hip).
+bool batch_pass_exponents(int log_m, std::vector<u32> &ex);
 
 // ---- large engine (ll_large.hip) ----------------------------------------------------------
 // Kernel arguments (by value).  Table pointers are device addresses.

@@ -75,6 +75,26 @@ struct RowSched {
     static constexpr int M3 = odd == 15 ? (pw > 1 ? pw : 0) : 0;
 };
 
+// Radix lists (R1 .. R5, 0 = no pass) of the two transforms, for the per-pass twiddle tables.
+template <int H>
+struct ColRadix {
+    typedef ColSched<H> S;
+    static constexpr int R1 = S::R1, R2 = S::R2, R3 = S::R3, R4 = S::R4, R5 = 0;
+};
+template <int W>
+struct RowRadix {
+    typedef RowSched<W> RS;
+    typedef Sched<RS::HYBRID ? 10 : ilog2c(W)> S;
+    static constexpr int R1 = RS::HYBRID ? 8 : S::R1;
+    static constexpr int R2 = RS::HYBRID ? RS::M1 : S::R2;
+    static constexpr int R3 = RS::HYBRID ? (RS::M2 ? RS::M2 : 8) : S::R3;
+    static constexpr int R4 = RS::HYBRID ? (RS::M2 ? (RS::M3 ? RS::M3 : 8) : 0) : S::R4;
+    static constexpr int R5 = RS::HYBRID ? (RS::M2 && RS::M3 ? 8 : 0) : 0;
+};
+
+// LDS pad shift of the large-engine transforms (one pad element per 8: radix-8 passes)
+constexpr int kPS = 3;
+
 template <bool IS_GL>
 struct ElemOf {
     typedef typename std::conditional<IS_GL, FGL, FC64>::type F;
@@ -217,15 +237,17 @@ LL_DEV void wait_flag(const unsigned *flag, u32 seq, unsigned *fault, int tid) {
     __syncthreads();
 }
 
-// Copy a LEN-entry twiddle table into LDS (all NT threads of the workgroup).
-template <class T, int LEN, int NT>
-LL_DEV void stage_table(T *dst, const void *src, int tid) {
-    const gptr<T> s = as_global<T>(src);
-#pragma unroll
-    for (int i = 0; i < (LEN + NT - 1) / NT; ++i) {
-        const int k = tid + NT * i;
-        if (LEN % NT == 0 || k < LEN) dst[k] = ld_table(s, (u32)k);
-    }
+// Per-pass twiddle tables of the H-point and W-point transforms, copied into LDS out of the full
+// tables (all NT threads of the workgroup).
+template <class T, int H, int NT>
+LL_DEV void stage_col_table(T *dst, const void *src, int tid) {
+    typedef ColRadix<H> RR;
+    stage_pass_table<T, NT, RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>(dst, src, tid);
+}
+template <class T, int W, int NT>
+LL_DEV void stage_row_table(T *dst, const void *src, int tid) {
+    typedef RowRadix<W> RR;
+    stage_pass_table<T, NT, RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>(dst, src, tid);
 }
 
 // ---- column kernels ---------------------------------------------------------------------------
@@ -246,7 +268,7 @@ struct Col {
     typedef typename E::WT WT;
     typedef ColSched<H> S;
     static constexpr int GH = H / kR, NT = kCW * GH, NBLK = W / kCW;
-    static constexpr int REG = H + H / 16 + (IS_GL ? 8 : 4);
+    static constexpr int REG = H + H / 8 + (IS_GL ? 8 : 4);
     static constexpr int NROW = kNRow;
     static constexpr int NWT = nwt(IS_GL);
     static_assert(H == NT * NROW, "row segments must split evenly over the workgroup");
@@ -299,9 +321,9 @@ LL_DEV void col_inverse(typename Col<IS_GL, W, H>::T (&x)[kR], typename Col<IS_G
     typedef typename C::S S;
     T *lds = lds_all + c * C::REG;
     if constexpr (S::MIXED)
-        mixed_transform<F, H, C::GH, kR, true, S::R1, S::R2, S::R3, S::R4>(x, lds, t, tw);
+        mixed_transform<F, H, C::GH, kR, true, S::R1, S::R2, S::R3, S::R4, true, kPS>(x, lds, t, tw);
     else
-        transform<F, H, C::GH, kR, S::R1, S::R2, S::R3, S::R4, true>(x, lds, t, tw);
+        transform<F, H, C::GH, kR, S::R1, S::R2, S::R3, S::R4, true, true, kPS>(x, lds, t, tw);
     const bool has_d0 = blk == 0 && c == 0 && t == 0;
 #pragma unroll
     for (int q = 0; q < kR; ++q) {
@@ -414,9 +436,9 @@ LL_DEV void col_forward(typename Col<IS_GL, W, H>::T *lds_all, TP tw,
     }
     T *lds = lds_all + c * C::REG;
     if constexpr (S::MIXED)
-        mixed_transform<F, H, C::GH, kR, false, S::R1, S::R2, S::R3, S::R4>(x, lds, t, tw);
+        mixed_transform<F, H, C::GH, kR, false, S::R1, S::R2, S::R3, S::R4, true, kPS>(x, lds, t, tw);
     else
-        transform<F, H, C::GH, kR, S::R1, S::R2, S::R3, S::R4, false>(x, lds, t, tw);
+        transform<F, H, C::GH, kR, S::R1, S::R2, S::R3, S::R4, false, true, kPS>(x, lds, t, tw);
     T *elem = reinterpret_cast<T *>(A.elem);
     const int w = blk * kCW + c;
 #pragma unroll
@@ -473,7 +495,7 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     col_weights<IS_GL, W, H>(wf, A.wfL, A.wfH, A, w, t);
     T tm[kR];
     col_twiddles<IS_GL, W, H>(tm, A.tmL, A.tmH, A.log_s, w, t);
-    stage_table<T, H, C::NT>(s_tw, A.twH, tid);
+    stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
 #pragma unroll
     for (int s = 0; s < kR; ++s) lds_all[(t + C::GH * s) * kCW + c] = x[s];
     __syncthreads();
@@ -499,7 +521,7 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
     col_load_inv<IS_GL, W, H>(x, tmi, A, w, t);
     WT wi[C::NWT];
     col_weights<IS_GL, W, H>(wi, A.wiL, A.wiH, A, w, t);
-    stage_table<T, H, C::NT>(s_tw, IS_GL ? A.twHi : A.twH, tid);
+    stage_col_table<T, H, C::NT>(s_tw, IS_GL ? A.twHi : A.twH, tid);
     CT cy[C::NROW];
     double eloc = 0.0;
     col_inverse<IS_GL, W, H>(x, lds_all, (const T *)s_tw, wi, A, blk, c, t, tid, cy, eloc);
@@ -542,8 +564,8 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fused(LargeArgs A) {
         s_seq = (u32)(atomicAdd(A.ticket, 1ull) / (unsigned long long)NBLK) + 1u;
         s_err = 0;
     }
-    stage_table<T, H, C::NT>(s_tw, A.twH, tid);
-    if constexpr (IS_GL) stage_table<T, H, C::NT>(s_twi, A.twHi, tid);
+    stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
+    if constexpr (IS_GL) stage_col_table<T, H, C::NT>(s_twi, A.twHi, tid);
     T tm[kR];
     col_twiddles<IS_GL, W, H>(tm, A.tmL, A.tmH, A.log_s, w, t);
     T x[kR];
@@ -582,10 +604,10 @@ template <class F, int W, int GW, bool INV, class TP>
 LL_DEV void row_transform(typename F::T (&x)[kR], typename F::T *lds, int t, TP tw) {
     typedef RowSched<W> RS;
     if constexpr (RS::HYBRID) {
-        hybrid_transform<F, W, GW, kR, INV, RS::M1, RS::M2, RS::M3>(x, lds, t, tw);
+        hybrid_transform<F, W, GW, kR, INV, RS::M1, RS::M2, RS::M3, true, kPS>(x, lds, t, tw);
     } else {
         typedef Sched<ilog2c(W)> S;
-        transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, INV>(x, lds, t, tw);
+        transform<F, W, GW, kR, S::R1, S::R2, S::R3, S::R4, INV, true, kPS>(x, lds, t, tw);
     }
 }
 
@@ -596,7 +618,7 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
     typedef typename E::F F;
     typedef typename E::T T;
     constexpr int GW = W / kR, NT = 2 * GW;
-    constexpr int REG = W + W / 16 + 4;
+    constexpr int REG = W + W / 8 + 4;
     __shared__ __attribute__((aligned(16))) T lds_all[2 * REG];
     __shared__ __attribute__((aligned(16))) T s_tw[W];
     __shared__ __attribute__((aligned(16))) T s_twi[IS_GL ? W : 1];
@@ -614,8 +636,8 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
 #pragma unroll
         for (int q = 0; q < kR; ++q) wk[q] = tw2<C64>(A.tmL, A.tmH, (u32)row + (u32)H * (u32)(t + GW * q), A.log_s);
     }
-    stage_table<T, W, NT>(s_tw, A.twW, tid);
-    if constexpr (IS_GL) stage_table<T, W, NT>(s_twi, A.twWi, tid);
+    stage_row_table<T, W, NT>(s_tw, A.twW, tid);
+    if constexpr (IS_GL) stage_row_table<T, W, NT>(s_twi, A.twWi, tid);
     row_transform<F, W, GW, false>(x, lds, t, (const T *)s_tw);
     if constexpr (IS_GL) {
 #pragma unroll
