@@ -46,7 +46,8 @@ struct Ctl {
 
 // LDS of the batch kernels: every group's residue buffer (padded) and the control block.
 // Referenced directly (not through pointer arguments) so every access is a ds_* instruction.
-constexpr int kSmemBytes = kWG * 8 * 16 * 17 / 16;
+// (a group of G threads holds M = G R elements, padded one per R: lds_pad_s<log2 R>)
+constexpr int kSmemBytes = kWG * 8 * 16 * 9 / 8;
 __shared__ __attribute__((aligned(16))) char g_smem[kSmemBytes];
 __shared__ Ctl g_ctl;
 // FP64 two-level weight tables of the class being run (ClassDesc::wtab)
@@ -112,7 +113,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     typedef typename F::W W;
     typedef typename std::conditional<IS_GL, u64, double>::type WT;
     constexpr int NG = kWG / G;
-    constexpr int REG = M + M / 16;
+    constexpr int PS = R == 8 ? 3 : 2;            // LDS pad shift: one pad element per R
+    static_assert(R == 8 || R == 4, "per-thread counts 4 and 8");
+    constexpr int REG = M + (M >> PS);
+    static_assert(kWG / G * REG * (int)sizeof(T) <= kSmemBytes, "LDS buffer too small");
     static_assert(NG <= kMaxGroups, "too many groups per workgroup");
 
     const int tid = threadIdx.x, g = tid / G, t = tid % G;
@@ -186,7 +190,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - pw) & nmask;
                 x[s] = cmk(x[s].re * wlds(e0, 0), x[s].im * wlds(e1, 0));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, false, true>(x, lds, t, twp);
+            transform<F, M, G, R, R1, R2, R3, R4, false, true, PS>(x, lds, t, twp);
             // ---- square the real spectrum in packed form (pairs k, M-k)
             const int tq = ll_opaque(t);
             __syncthreads();
@@ -205,7 +209,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const C64 Z2 = csqr(x[q]);
                 x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, true, true>(x, lds, t, twp);
+            transform<F, M, G, R, R1, R2, R3, R4, true, true, PS>(x, lds, t, twp);
             // ---- unweight, round, error, -2, carry within the pair
             double cout[R];
             const int tu = ll_opaque(t);
@@ -278,10 +282,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const u32 e = (0u - j * pw) & nmask;
                 x[s] = gl_mul(gl_from_i64((i64)x[s]), ld_table(wf, e));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, false, true>(x, lds, t, twp);
+            transform<F, M, G, R, R1, R2, R3, R4, false, true, PS>(x, lds, t, twp);
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = gl_sqr(x[q]);
-            transform<F, M, G, R, R1, R2, R3, R4, true, true>(x, lds, t, twpi);
+            transform<F, M, G, R, R1, R2, R3, R4, true, true, PS>(x, lds, t, twpi);
             i64 cout[R];
             const int tu = ll_opaque(t);
             const u32 pu = ll_opaque(p);

@@ -209,29 +209,42 @@ LL_DEV void dft_r(typename F::T *x) {
 // inverse transform the C64 path conjugates, the GL path is handed the inverse table).
 // FIRST: inputs come from registers;  LAST: outputs stay in registers.
 // OFF >= 0: `tw` is a per-pass table (PassTable) and this pass's entries start at OFF.
-// PS: LDS pad shift (lds_pad_s).
+// PS: LDS pad shift (lds_pad_s), P = 2^PS.
+//
+// Addressing: G is a multiple of L (L <= M / R = G) so k = (t + G b) mod L = t mod L, and the
+// padded LDS index of every access is a per-thread base plus a compile-time constant (an
+// immediate offset): for C a multiple of P, pad(t + C) = pad(t) + C (P + 1) / P.
 template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV, int OFF = -1, int PS = 4,
           class TP>
 LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(R % r == 0, "radix must divide the per-thread count");
     static_assert(!FIRST || r == R, "first pass must use the full per-thread radix");
     static_assert(!LAST || (r == R && L * r == M), "last pass must use the full radix");
+    static_assert(G % L == 0, "group size must be a multiple of the pass span");
     constexpr int NB = R / r;
     constexpr int STRIDE = M / r;
+    constexpr int P = 1 << PS;
     t = ll_opaque(t);
     if constexpr (!FIRST) {
         __syncthreads();
+        if constexpr (G % P == 0 && STRIDE % P == 0) {
+            const int rb = lds_pad_s<PS>(t);
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+            for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int s = 0; s < r; ++s) x[b * r + s] = lds[lds_pad_s<PS>(t + G * b + s * STRIDE)];
+                for (int s = 0; s < r; ++s) x[b * r + s] = lds[rb + (G * b + s * STRIDE) / P * (P + 1)];
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int s = 0; s < r; ++s) x[b * r + s] = lds[lds_pad_s<PS>(t + G * b + s * STRIDE)];
+        }
     }
     if constexpr (L > 1) {
+        const int k = t % L;
+        constexpr int TSTEP = M / (r * L);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            const int i = t + G * b;
-            const int k = i % L;
-            constexpr int TSTEP = M / (r * L);
 #pragma unroll
             for (int s = 1; s < r; ++s) {
                 const u32 e = OFF >= 0 ? (u32)(OFF + (s - 1) * L + k) : (u32)(s * k * TSTEP);
@@ -243,12 +256,30 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
     for (int b = 0; b < NB; ++b) dft_r<F, r, INV>(&x[b * r]);
     if constexpr (!LAST) {
         __syncthreads();
+        // output q of butterfly i = t + G b goes to (i - k) r + k + q L
+        if constexpr (L == 1 && r == P) {
+            // i r + q, q < P: padded i (r + 1) + q
+            const int wb = t * (r + 1);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int i = t + G * b;
-            const int k = i % L;
+            for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int q = 0; q < r; ++q) lds[lds_pad_s<PS>((i - k) * r + k + q * L)] = x[b * r + q];
+                for (int q = 0; q < r; ++q) lds[wb + G * b * (r + 1) + q] = x[b * r + q];
+        } else if constexpr (L % P == 0) {
+            // (t - k) r, G b r and q L are multiples of P; k < L adds k + k / P
+            const int k = t % L;
+            const int wb = (t - k) * r / P * (P + 1) + k + (k >> PS);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < r; ++q) lds[wb + (G * b * r + q * L) / P * (P + 1)] = x[b * r + q];
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int i = t + G * b;
+                const int k = i % L;
+#pragma unroll
+                for (int q = 0; q < r; ++q) lds[lds_pad_s<PS>((i - k) * r + k + q * L)] = x[b * r + q];
+            }
         }
     }
 }
