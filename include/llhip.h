@@ -76,6 +76,26 @@ double llh_get_fft_bits_per_word(void);
 int llh_set_batch_bits_per_word(double bits);
 double llh_get_batch_bits_per_word(void);
 
+/* Length families the batched FP64 engine may pick: bit 0 powers of two (always on), bit 1
+ * 3 * 2^k, bit 2 5 * 2^k, bit 3 7 * 2^k (default 0xF).  No reference counterpart: the reference's
+ * GPU batch runs powers of two only (cuda/ll_ntt.cu:176-180). */
+int llh_set_batch_families(int mask);
+int llh_get_batch_families(void);
+
+/* Measured cost (device microseconds per exponent-squaring at full occupancy) of batched
+ * length n; the engine runs, of the lengths within the bits-per-digit cap (up to twice the
+ * shortest), the cheapest.  Defaults: the autotuner's table for MI355X; us = 0 forgets a cost.
+ * No reference counterpart (the reference sizes each exponent by the cap alone,
+ * lucaslehmer.cpp:244-254). */
+int llh_set_batch_cost(uint64_t n, int transform, double us);
+double llh_get_batch_cost(uint64_t n, int transform);
+
+/* llh_batch with every exponent run at length n_force (0 = the engine's choice) and flags
+ * bit 0 = no rollback.  No reference counterpart: the autotuner's entry point. */
+int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, uint64_t n_force, int flags,
+                int32_t *verdict, uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used,
+                int32_t *status);
+
 /* Many exponents in one persistent launch.  Replaces lucas_lehmer_batch cuda/ll_ntt.cu:259-267
  * (and the per-thread engine calls of PerfNumMultiCLL.py:189-193).  max_iters = 0 runs the full
  * p-2 squarings.  Outputs are indexed like ps[] (the reference returns them grouped by length);

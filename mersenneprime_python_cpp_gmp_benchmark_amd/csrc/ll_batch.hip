@@ -1,19 +1,24 @@
 // ll_batch.hip -- the batched, LDS-resident Lucas-Lehmer engine: every exponent of a list
-// in ONE persistent launch, whatever its transform length.
+// in a few persistent launches, whatever its transform length.
 //
 // Counterpart of cuda/ll_ntt.cu:74-167 (ll_kernel, one block per exponent and one launch
 // per length group) and of the FP64 loop lucaslehmer.cpp:281-311, re-designed for gfx950:
 //
-//  * A workgroup is 256 threads.  A size class with M transform elements is run by groups
-//    of G = M/R threads (R elements per thread, R = 16 for M >= 256), so one workgroup holds
-//    256/G exponents of that class at once, all stepping in lock-step.
+//  * Size classes: a class has M transform elements (FP64: N = 2M digits, GL: N = M) and is
+//    run by groups of G = M/R threads (R elements per thread, R = 8; R = 4 for M < 64).  M is a
+//    power of two (16 .. 4096) or 8 m 8 with m = 3, 5, 7 times a power of two (FP64 only).
+//  * Families: the classes of one odd factor share a workgroup size that their G divide
+//    (powers of two 512, 3: 384, 5: 320, 7: 448), one persistent kernel per family.  A
+//    workgroup holds WG/G exponents of a class at once, all stepping in lock-step.
 //  * Workgroups pull exponents from one queue per class (an atomic counter over items the
 //    host sorted longest-first), classes in descending length order, then the one-word lane
-//    (p < 64) -- one launch for the whole list, no per-length launches.
+//    (p < 64).  The families' launches run on separate streams, so one family's tail is
+//    back-filled by the next.
 //  * The residue lives in registers in a "column" layout (thread t holds elements t + G s);
 //    only the transform's middle hand-offs, the real-FFT pairing and the carry hand-off go
 //    through LDS.  Nothing touches HBM between the first and the last squaring except the
-//    read-only twiddle and weight tables (L2-resident).
+//    read-only twiddle tables (L2-resident, per-pass order so every wave load is contiguous);
+//    the FP64 digit weights come from two-level tables staged in LDS.
 //  * FP64 (FC64): the N weighted real digits are packed as N/2 complex numbers
 //    z_j = y_{2j} + i y_{2j+1}; after the forward complex FFT the spectrum of the real
 //    signal is squared in packed form:  Z'_k = Z_k^2 - (1 + w_M^k) (Z_k - conj Z_{M-k})^2 / 4,
@@ -32,9 +37,8 @@
 
 namespace {
 
-constexpr int kWG = 512;
 constexpr u32 kNone = 0xFFFFFFFFu;
-constexpr double kErrLimit = 0.4;   // round-off above this aborts the exponent (rollback at 2N)
+constexpr double kErrLimit = 0.4;   // round-off above this aborts the exponent (rollback longer)
 
 constexpr int kMaxGroups = 128;
 
@@ -44,42 +48,77 @@ struct Ctl {
     unsigned long long err[kMaxGroups];
 };
 
-// LDS of the batch kernels: every group's residue buffer (padded) and the control block.
-// Referenced directly (not through pointer arguments) so every access is a ds_* instruction.
-// (a group of G threads holds M = G R elements, padded one per R: lds_pad_s<log2 R>)
-constexpr int kSmemBytes = kWG * 8 * 16 * 9 / 8;
-__shared__ __attribute__((aligned(16))) char g_smem[kSmemBytes];
+// Size classes.  Powers of two: radices R1 .. R4 (first = last = R; 0 = no pass).  Others:
+// the hybrid schedule 8, H1, [H2], 8 (H1 odd; register radix-8 first and last passes, the
+// middle ones through LDS).  WG: the family's workgroup size.
+struct Cls {
+    int M, G, R, WG;
+    int R1, R2, R3, R4;
+    int H1, H2;
+};
+constexpr Cls kCls[] = {
+    {0, 0, 0, 512, 0, 0, 0, 0, 0, 0},          // 0: one-word lane (p < 64)
+    {16, 4, 4, 512, 4, 4, 0, 0, 0, 0},         // 1
+    {32, 8, 4, 512, 4, 2, 4, 0, 0, 0},         // 2
+    {64, 8, 8, 512, 8, 8, 0, 0, 0, 0},         // 3
+    {128, 16, 8, 512, 8, 2, 8, 0, 0, 0},       // 4
+    {256, 32, 8, 512, 8, 4, 8, 0, 0, 0},       // 5
+    {512, 64, 8, 512, 8, 8, 8, 0, 0, 0},       // 6
+    {1024, 128, 8, 512, 8, 2, 8, 8, 0, 0},     // 7
+    {2048, 256, 8, 512, 8, 4, 8, 8, 0, 0},     // 8
+    {4096, 512, 8, 512, 8, 8, 8, 8, 0, 0},     // 9
+    {192, 24, 8, 384, 0, 0, 0, 0, 3, 0},       // 10
+    {384, 48, 8, 384, 0, 0, 0, 0, 3, 2},       // 11
+    {768, 96, 8, 384, 0, 0, 0, 0, 3, 4},       // 12
+    {1536, 192, 8, 384, 0, 0, 0, 0, 3, 8},     // 13
+    {3072, 384, 8, 384, 0, 0, 0, 0, 3, 16},    // 14
+    {320, 40, 8, 320, 0, 0, 0, 0, 5, 0},       // 15
+    {640, 80, 8, 320, 0, 0, 0, 0, 5, 2},       // 16
+    {1280, 160, 8, 320, 0, 0, 0, 0, 5, 4},     // 17
+    {2560, 320, 8, 320, 0, 0, 0, 0, 5, 8},     // 18
+    {448, 56, 8, 448, 0, 0, 0, 0, 7, 0},       // 19
+    {896, 112, 8, 448, 0, 0, 0, 0, 7, 2},      // 20
+    {1792, 224, 8, 448, 0, 0, 0, 0, 7, 4},     // 21
+    {3584, 448, 8, 448, 0, 0, 0, 0, 7, 8},     // 22
+};
+constexpr int kNumCls = (int)(sizeof(kCls) / sizeof(kCls[0]));
+constexpr int kFamilyWG[4] = {512, 384, 320, 448};
+
+// radix list of class C for the per-pass twiddle table (PassTable<R1 .. R5>)
+template <int C>
+struct ClsRadix {
+    static constexpr Cls K = kCls[C];
+    static constexpr bool P2 = K.H1 == 0;
+    static constexpr int R1 = P2 ? K.R1 : 8;
+    static constexpr int R2 = P2 ? K.R2 : K.H1;
+    static constexpr int R3 = P2 ? K.R3 : (K.H2 ? K.H2 : 8);
+    static constexpr int R4 = P2 ? K.R4 : (K.H2 ? 8 : 0);
+    typedef PassTable<R1, R2, R3, R4, 0> PT;
+    static_assert(PT::M == K.M, "schedule must cover the class length");
+};
+
+// LDS of a family's kernel: every group's residue buffer (M + M/R padded elements of 16 bytes
+// at most, WG/G groups) and the control block.  Referenced directly (not through pointer
+// arguments) so every access is a ds_* instruction.
+template <int WG>
+__shared__ __attribute__((aligned(16))) char g_smem[WG * 8 * 16 * 9 / 8];
 __shared__ Ctl g_ctl;
 // FP64 two-level weight tables of the class being run (ClassDesc::wtab)
 __shared__ double g_wtab[4 * llh::kWtabSlot];
-
-// Radix schedule of each size class (index log2 M): M, G = M / R threads per exponent, R
-// elements per thread, radices R1 .. R4 (first = last = R; 0 = no pass).
-struct Sch {
-    int M, G, R, R1, R2, R3, R4;
-};
-constexpr Sch kSch[13] = {
-    {0, 0, 0, 0, 0, 0, 0},          {0, 0, 0, 0, 0, 0, 0},       {0, 0, 0, 0, 0, 0, 0},
-    {0, 0, 0, 0, 0, 0, 0},          {16, 4, 4, 4, 4, 0, 0},      {32, 8, 4, 4, 2, 4, 0},
-    {64, 8, 8, 8, 8, 0, 0},         {128, 16, 8, 8, 2, 8, 0},    {256, 32, 8, 8, 4, 8, 0},
-    {512, 64, 8, 8, 8, 8, 0},       {1024, 128, 8, 8, 2, 8, 8},  {2048, 256, 8, 8, 4, 8, 8},
-    {4096, 512, 8, 8, 8, 8, 8},
-};
 
 // Exact canonical residue of N near-balanced digits (natural order, i64, in LDS):
 // full carry propagation with wrap (2^p == 1), then Res64 = low 64 bits of the value in
 // [0, 2^p - 1), with the all-ones pattern (== M_p == 0) folded to zero.
 // Restates the GMP rebuild of lucaslehmer.cpp:312-323 / cuda/ll_ntt.cu:238-251 without GMP.
-__device__ int normalize_res64(i64 *d, u32 p, int log_n, u64 &res, int &zero) {
-    const u32 N = 1u << log_n;
+__device__ int normalize_res64(i64 *d, u32 p, u32 N, u64 &res, int &zero) {
+    const DigitWalk DW{N, p % N, p / N};
     i64 carry = 0;
     for (int round = 0; round < 4; ++round) {
-        u64 cj = 0;
+        u32 r = 0;
         for (u32 j = 0; j < N; ++j) {
             if (round > 0 && carry == 0) break;
-            u64 cn = ((u64)(j + 1) * p + N - 1) >> log_n;
-            int b = (int)(cn - cj);
-            cj = cn;
+            const int b = DW.bits(r);
+            r = DW.next(r);
             i64 v = d[j] + carry;
             d[j] = v & (((i64)1 << b) - 1);
             carry = v >> b;
@@ -87,42 +126,56 @@ __device__ int normalize_res64(i64 *d, u32 p, int log_n, u64 &res, int &zero) {
         if (carry == 0) break;
     }
     if (carry != 0) return -1;
-    u64 r = 0, cj = 0;
+    u64 res_ = 0, cj = 0;
     int z = 1, ones = 1;
+    u32 r = 0;
     for (u32 j = 0; j < N; ++j) {
-        u64 cn = ((u64)(j + 1) * p + N - 1) >> log_n;
-        int b = (int)(cn - cj);
+        const int b = DW.bits(r);
+        r = DW.next(r);
         u64 dj = (u64)d[j];
         if (dj) z = 0;
         if (dj != (((u64)1 << b) - 1)) ones = 0;
-        if (cj < 64) r |= dj << cj;
-        cj = cn;
+        if (cj < 64) res_ |= dj << cj;
+        cj += (u64)b;
     }
-    if (ones) { r = 0; z = 1; }
-    res = r;
+    if (ones) { res_ = 0; z = 1; }
+    res = res_;
     zero = z;
     return 0;
 }
 
-template <bool IS_GL, int M, int G, int R, int R1, int R2, int R3, int R4>
+// Bits and weight exponents of the two digits 2j, 2j + 1 of FP64 element j.
+struct Pair {
+    int b0, b1;
+    u32 e0, e1;
+};
+
+template <bool IS_GL, int WG, int C>
 __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, const llh::BatchItem *__restrict__ items,
                           llh::BatchResult *__restrict__ out, u32 *counter) {
     Ctl &ctl = g_ctl;
+    constexpr Cls K = kCls[C];
+    constexpr int M = K.M, G = K.G, R = K.R;
+    constexpr bool P2 = K.H1 == 0;
+    static_assert(!IS_GL || P2, "the Goldilocks classes are powers of two");
+    static_assert(K.WG == WG, "class run by another family's kernel");
     typedef typename std::conditional<IS_GL, FGL, FC64>::type F;
     typedef typename F::T T;
     typedef typename F::W W;
     typedef typename std::conditional<IS_GL, u64, double>::type WT;
-    constexpr int NG = kWG / G;
+    constexpr int NG = WG / G;
     constexpr int PS = R == 8 ? 3 : 2;            // LDS pad shift: one pad element per R
     static_assert(R == 8 || R == 4, "per-thread counts 4 and 8");
+    static_assert(WG % G == 0, "groups must tile the workgroup");
     constexpr int REG = M + (M >> PS);
-    static_assert(kWG / G * REG * (int)sizeof(T) <= kSmemBytes, "LDS buffer too small");
     static_assert(NG <= kMaxGroups, "too many groups per workgroup");
+    static_assert(NG * REG * (int)sizeof(T) <= (int)sizeof(g_smem<WG>), "LDS buffer too small");
+    constexpr u32 N = IS_GL ? (u32)M : 2u * (u32)M;
 
     const int tid = threadIdx.x, g = tid / G, t = tid % G;
-    T *lds = reinterpret_cast<T *>(g_smem) + g * REG;
-    const int log_n = cd.log_n;
-    const u32 N = 1u << log_n, nmask = N - 1;
+    T *lds = reinterpret_cast<T *>(g_smem<WG>) + g * REG;
+    const int log_n = cd.log_n;                   // powers of two only
+    const u32 nmask = N - 1;
     const gptr<W> tw = as_global<W>(cd.tw);
     const gptr<W> twp = as_global<W>(cd.twp);
     const gptr<W> twpi = IS_GL ? as_global<W>(cd.twpi) : twp;
@@ -131,12 +184,50 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     const u32 smask = (1u << cd.log_s) - 1;
     const int log_s = cd.log_s;
     if constexpr (!IS_GL) {
-        static_assert(4 * llh::kWtabSlot <= kWG, "one weight entry per thread");
-        if (tid < 4 * llh::kWtabSlot) g_wtab[tid] = ld_table(as_global<double>(cd.wtab), (u32)tid);
+#pragma unroll
+        for (int i = tid; i < 4 * llh::kWtabSlot; i += WG) g_wtab[i] = ld_table(as_global<double>(cd.wtab), (u32)i);
     }
     // weight of exponent e (forward: slots 0, 1; inverse: slots 2, 3)
     auto wlds = [&](u32 e, int inv) {
         return g_wtab[(2 * inv) * llh::kWtabSlot + (e & smask)] * g_wtab[(2 * inv + 1) * llh::kWtabSlot + (e >> log_s)];
+    };
+    // Digits 2j, 2j + 1 of element j = t + G s, s = 0 .. R-1, in order: powers of two from
+    // shifts and masks, any N from the residue walk (r = 2 j p mod N, step 2 G p mod N).
+    struct Layout {
+        u32 p;
+        DigitWalk dw;
+        u32 step, r;
+    };
+    auto layout = [&](u32 pp, int tt) {
+        Layout L;
+        L.p = pp;
+        if constexpr (!P2) {
+            L.dw = DigitWalk{N, pp % N, pp / N};
+            L.step = (2u * (u32)G * pp) % N;
+            L.r = (2u * (u32)tt * pp) % N;
+        }
+        return L;
+    };
+    auto pair_at = [&](Layout &L, int tt, int s) {
+        Pair q;
+        if constexpr (P2) {
+            const u32 j = (u32)tt + (u32)G * (u32)s;
+            const u32 dp0 = 2u * j * L.p;
+            const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + L.p + N - 1) >> log_n,
+                      c2 = (dp0 + 2u * L.p + N - 1) >> log_n;
+            q.b0 = (int)(c1 - c0);
+            q.b1 = (int)(c2 - c1);
+            q.e0 = (0u - dp0) & nmask;
+            q.e1 = (0u - dp0 - L.p) & nmask;
+        } else {
+            const u32 r0 = L.r, r1 = L.dw.next(r0);
+            q.b0 = L.dw.bits(r0);
+            q.b1 = L.dw.bits(r1);
+            q.e0 = L.dw.wexp(r0);
+            q.e1 = L.dw.wexp(r1);
+            L.r = L.dw.add(r0, L.step);
+        }
+        return q;
     };
 
     u32 item = kNone, p = N + 1;
@@ -181,16 +272,19 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 
         if constexpr (!IS_GL) {
             // ---- weight: z_j = x_{2j} a_{2j} + i x_{2j+1} a_{2j+1}
-            const int tw0 = ll_opaque(t);
-            const u32 pw = ll_opaque(p);
+            {
+                const int tw0 = ll_opaque(t);
+                Layout L = layout(ll_opaque(p), tw0);
 #pragma unroll
-            for (int s = 0; s < R; ++s) {
-                const u32 j = tw0 + G * s;
-                const u32 dp0 = 2u * j * pw;
-                const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - pw) & nmask;
-                x[s] = cmk(x[s].re * wlds(e0, 0), x[s].im * wlds(e1, 0));
+                for (int s = 0; s < R; ++s) {
+                    const Pair q = pair_at(L, tw0, s);
+                    x[s] = cmk(x[s].re * wlds(q.e0, 0), x[s].im * wlds(q.e1, 0));
+                }
             }
-            transform<F, M, G, R, R1, R2, R3, R4, false, true, PS>(x, lds, t, twp);
+            if constexpr (P2)
+                transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, false, true, PS>(x, lds, t, twp);
+            else
+                hybrid_transform<F, M, G, R, false, K.H1, K.H2, 0, true, PS>(x, lds, t, twp);
             // ---- square the real spectrum in packed form (pairs k, M-k)
             const int tq = ll_opaque(t);
             __syncthreads();
@@ -200,7 +294,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int k = tq + G * q;
-                const int km = (M - k) & (M - 1);
+                const int km = k == 0 ? 0 : M - k;
                 const C64 zm = lds[lds_pad(km)];
                 const C64 D = csub(x[q], cconj(zm));
                 const C64 w = ld_table(tw, (u32)k);
@@ -209,30 +303,31 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const C64 Z2 = csqr(x[q]);
                 x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, true, true, PS>(x, lds, t, twp);
+            if constexpr (P2)
+                transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, true, true, PS>(x, lds, t, twp);
+            else
+                hybrid_transform<F, M, G, R, true, K.H1, K.H2, 0, true, PS>(x, lds, t, twp);
             // ---- unweight, round, error, -2, carry within the pair
             double cout[R];
-            const int tu = ll_opaque(t);
-            const u32 pu = ll_opaque(p);
+            {
+                const int tu = ll_opaque(t);
+                Layout L = layout(ll_opaque(p), tu);
 #pragma unroll
-            for (int s = 0; s < R; ++s) {
-                const u32 j = tu + G * s;
-                const u32 dp0 = 2u * j * pu;
-                const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + pu + N - 1) >> log_n,
-                          c2 = (dp0 + 2u * pu + N - 1) >> log_n;
-                const int b0 = (int)(c1 - c0), b1 = (int)(c2 - c1);
-                const u32 e0 = (0u - dp0) & nmask, e1 = (0u - dp0 - pu) & nmask;
-                const double v0 = x[s].re * wlds(e0, 1), v1 = x[s].im * wlds(e1, 1);
-                double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
-                eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
-                if (j == 0 && has) r0 -= 2.0;
-                const double cA = floor(__fma_rn(r0, ldexp(1.0, -b0), 0.5));
-                const double x0 = __fma_rn(-cA, ldexp(1.0, b0), r0);
-                r1 += cA;
-                const double cB = floor(__fma_rn(r1, ldexp(1.0, -b1), 0.5));
-                const double x1 = __fma_rn(-cB, ldexp(1.0, b1), r1);
-                x[s] = cmk(x0, x1);
-                cout[s] = cB;
+                for (int s = 0; s < R; ++s) {
+                    const u32 j = tu + G * s;
+                    const Pair q = pair_at(L, tu, s);
+                    const double v0 = x[s].re * wlds(q.e0, 1), v1 = x[s].im * wlds(q.e1, 1);
+                    double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
+                    eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
+                    if (j == 0 && has) r0 -= 2.0;
+                    const double cA = floor(__fma_rn(r0, ldexp(1.0, -q.b0), 0.5));
+                    const double x0 = __fma_rn(-cA, ldexp(1.0, q.b0), r0);
+                    r1 += cA;
+                    const double cB = floor(__fma_rn(r1, ldexp(1.0, -q.b1), 0.5));
+                    const double x1 = __fma_rn(-cB, ldexp(1.0, q.b1), r1);
+                    x[s] = cmk(x0, x1);
+                    cout[s] = cB;
+                }
             }
             // ---- carry hand-offs: element j takes the carry out of element j-1 (wrap at 0).
             // Round 1 moves the large carries and re-splits both digits of the pair; round 2
@@ -241,27 +336,26 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             // ~sqrt(N)/4, feed back into the next squaring and grow).
             double *ldsd = reinterpret_cast<double *>(lds);
             const int tc = ll_opaque(t);
-            const u32 pc = ll_opaque(p);
             __syncthreads();
 #pragma unroll
             for (int s = 0; s < R; ++s) ldsd[tc + G * s] = cout[s];
             __syncthreads();
+            {
+                Layout L = layout(ll_opaque(p), tc);
 #pragma unroll
-            for (int s = 0; s < R; ++s) {
-                const u32 j = tc + G * s;
-                const double cin = ldsd[(j - 1) & (M - 1)];
-                const u32 dp0 = 2u * j * pc;
-                const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + pc + N - 1) >> log_n,
-                          c2 = (dp0 + 2u * pc + N - 1) >> log_n;
-                const int b0 = (int)(c1 - c0), b1 = (int)(c2 - c1);
-                double x0 = x[s].re + cin;
-                const double cC = floor(__fma_rn(x0, ldexp(1.0, -b0), 0.5));
-                x0 = __fma_rn(-cC, ldexp(1.0, b0), x0);
-                double x1 = x[s].im + cC;
-                const double cD = floor(__fma_rn(x1, ldexp(1.0, -b1), 0.5));
-                x1 = __fma_rn(-cD, ldexp(1.0, b1), x1);
-                x[s] = cmk(x0, x1);
-                cout[s] = cD;
+                for (int s = 0; s < R; ++s) {
+                    const u32 j = tc + G * s;
+                    const double cin = ldsd[j == 0 ? M - 1 : j - 1];
+                    const Pair q = pair_at(L, tc, s);
+                    double x0 = x[s].re + cin;
+                    const double cC = floor(__fma_rn(x0, ldexp(1.0, -q.b0), 0.5));
+                    x0 = __fma_rn(-cC, ldexp(1.0, q.b0), x0);
+                    double x1 = x[s].im + cC;
+                    const double cD = floor(__fma_rn(x1, ldexp(1.0, -q.b1), 0.5));
+                    x1 = __fma_rn(-cD, ldexp(1.0, q.b1), x1);
+                    x[s] = cmk(x0, x1);
+                    cout[s] = cD;
+                }
             }
             __syncthreads();
 #pragma unroll
@@ -270,7 +364,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
             for (int s = 0; s < R; ++s) {
                 const u32 j = tc + G * s;
-                x[s].re += ldsd[(j - 1) & (M - 1)];
+                x[s].re += ldsd[j == 0 ? M - 1 : j - 1];
             }
         } else {
             // ---- Goldilocks: weight, NTT, square, inverse NTT
@@ -282,10 +376,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const u32 e = (0u - j * pw) & nmask;
                 x[s] = gl_mul(gl_from_i64((i64)x[s]), ld_table(wf, e));
             }
-            transform<F, M, G, R, R1, R2, R3, R4, false, true, PS>(x, lds, t, twp);
+            transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, false, true, PS>(x, lds, t, twp);
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = gl_sqr(x[q]);
-            transform<F, M, G, R, R1, R2, R3, R4, true, true, PS>(x, lds, t, twpi);
+            transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, true, true, PS>(x, lds, t, twpi);
             i64 cout[R];
             const int tu = ll_opaque(t);
             const u32 pu = ll_opaque(p);
@@ -368,7 +462,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 } else {
                     u64 res = 0;
                     int zero = 0;
-                    r.status = normalize_res64(dig, p, log_n, res, zero) == 0 ? llh::kStOk : llh::kStNotRun;
+                    r.status = normalize_res64(dig, p, N, res, zero) == 0 ? llh::kStOk : llh::kStNotRun;
                     r.res64 = res;
                     r.verdict = zero;
                 }
@@ -420,74 +514,130 @@ __device__ void run_tiny(const llh::ClassDesc &cd, const llh::BatchItem *__restr
     }
 }
 
-template <bool IS_GL>
-__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 4)))
+// class C's branch of a family kernel (nothing for classes of other families or, for GL,
+// lengths with an odd factor)
+template <bool IS_GL, int WG, int C>
+__device__ __forceinline__ void run_if(const llh::ClassDesc &cd, const llh::BatchItem *__restrict__ items,
+                                       llh::BatchResult *__restrict__ out, u32 *counter) {
+    if constexpr (C > 0 && kCls[C].WG == WG && (!IS_GL || kCls[C].H1 == 0))
+        run_class<IS_GL, WG, C>(cd, items, out, counter);
+}
+
+template <bool IS_GL, int WG>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG == 384 ? 3 : 4, WG == 384 ? 3 : 4)))
 ll_batch_kernel(const llh::ClassDesc *__restrict__ classes, int ncls,
                 const llh::BatchItem *__restrict__ items, llh::BatchResult *__restrict__ out,
                 u32 *counters) {
     for (int c = 0; c < ncls; ++c) {
         const llh::ClassDesc cd = classes[c];
-        switch (cd.log_m) {
-#define LL_CLASS(L)                                                                                       \
-    case L:                                                                                               \
-        run_class<IS_GL, kSch[L].M, kSch[L].G, kSch[L].R, kSch[L].R1, kSch[L].R2, kSch[L].R3, kSch[L].R4>( \
-            cd, items, out, counters + c);                                                                \
-        break;
-            LL_CLASS(12) LL_CLASS(11) LL_CLASS(10) LL_CLASS(9) LL_CLASS(8) LL_CLASS(7) LL_CLASS(6)
-            LL_CLASS(5) LL_CLASS(4)
+        switch (cd.cls) {
+#define LL_CLASS(C) \
+    case C: run_if<IS_GL, WG, C>(cd, items, out, counters + c); break;
+            LL_CLASS(1) LL_CLASS(2) LL_CLASS(3) LL_CLASS(4) LL_CLASS(5) LL_CLASS(6) LL_CLASS(7)
+            LL_CLASS(8) LL_CLASS(9) LL_CLASS(10) LL_CLASS(11) LL_CLASS(12) LL_CLASS(13) LL_CLASS(14)
+            LL_CLASS(15) LL_CLASS(16) LL_CLASS(17) LL_CLASS(18) LL_CLASS(19) LL_CLASS(20) LL_CLASS(21)
+            LL_CLASS(22)
 #undef LL_CLASS
-            case 0: run_tiny(cd, items, out, counters + c); break;
+            case 0:
+                if constexpr (WG == 512) run_tiny(cd, items, out, counters + c);
+                break;
             default: break;
         }
         __syncthreads();
     }
+}
+static_assert(kNumCls == 23, "the class switch lists every class");
+
+template <int C>
+static void pass_exponents(std::vector<u32> &ex) {
+    typedef typename ClsRadix<C>::PT PT;
+    ex.resize(PT::SIZE);
+    for (int e = 0; e < PT::SIZE; ++e) ex[e] = PT::exponent(e);
 }
 
 }  // namespace
 
 namespace llh {
 
-// Launch the persistent batch kernel on `stream`; grid = resident workgroups.
-hipError_t launch_batch(int transform, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
-                        BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream) {
-    if (transform == kNTT)
-        hipLaunchKernelGGL(ll_batch_kernel<true>, dim3(grid), dim3(kWG), 0, stream, d_classes, ncls,
-                           d_items, d_out, d_counters);
-    else
-        hipLaunchKernelGGL(ll_batch_kernel<false>, dim3(grid), dim3(kWG), 0, stream, d_classes, ncls,
-                           d_items, d_out, d_counters);
-    return hipGetLastError();
-}
+int batch_num_classes() { return kNumCls; }
 
-template <int L>
-static void pass_exponents(std::vector<u32> &ex) {
-    typedef PassTable<kSch[L].R1, kSch[L].R2, kSch[L].R3, kSch[L].R4, 0> PT;
-    static_assert(PT::M == kSch[L].M, "schedule must cover the class length");
-    ex.resize(PT::SIZE);
-    for (int e = 0; e < PT::SIZE; ++e) ex[e] = PT::exponent(e);
+// class of transform length M (0 = none)
+int batch_class_of_m(u64 M) {
+    for (int c = 1; c < kNumCls; ++c)
+        if ((u64)kCls[c].M == M) return c;
+    return 0;
 }
+u64 batch_class_m(int c) { return c > 0 && c < kNumCls ? (u64)kCls[c].M : 0; }
+bool batch_class_pow2(int c) { return c > 0 && c < kNumCls && kCls[c].H1 == 0; }
+int batch_class_family(int c) {
+    const int wg = c >= 0 && c < kNumCls ? kCls[c].WG : 512;
+    for (int f = 0; f < 4; ++f)
+        if (kFamilyWG[f] == wg) return f;
+    return 0;
+}
+int batch_class_groups(int c) { return c > 0 && c < kNumCls ? kCls[c].WG / kCls[c].G : 1; }
 
-// Full-table exponent of every entry of the class's per-pass twiddle table.
-bool batch_pass_exponents(int log_m, std::vector<u32> &ex) {
-    switch (log_m) {
-        case 4: pass_exponents<4>(ex); return true;
-        case 5: pass_exponents<5>(ex); return true;
-        case 6: pass_exponents<6>(ex); return true;
-        case 7: pass_exponents<7>(ex); return true;
-        case 8: pass_exponents<8>(ex); return true;
-        case 9: pass_exponents<9>(ex); return true;
-        case 10: pass_exponents<10>(ex); return true;
-        case 11: pass_exponents<11>(ex); return true;
-        case 12: pass_exponents<12>(ex); return true;
+// Full-table exponent of every entry of class c's per-pass twiddle table.
+bool batch_pass_exponents(int c, std::vector<u32> &ex) {
+    switch (c) {
+#define LL_PX(C) \
+    case C: pass_exponents<C>(ex); return true;
+        LL_PX(1) LL_PX(2) LL_PX(3) LL_PX(4) LL_PX(5) LL_PX(6) LL_PX(7) LL_PX(8) LL_PX(9) LL_PX(10) LL_PX(11)
+        LL_PX(12) LL_PX(13) LL_PX(14) LL_PX(15) LL_PX(16) LL_PX(17) LL_PX(18) LL_PX(19) LL_PX(20) LL_PX(21)
+        LL_PX(22)
+#undef LL_PX
         default: return false;
     }
 }
 
-int batch_blocks_per_cu(int transform) {
+int batch_family_wg(int family) { return kFamilyWG[family & 3]; }
+
+// Launch family `family`'s persistent batch kernel on `stream`; grid = resident workgroups.
+hipError_t launch_batch(int transform, int family, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
+                        BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream) {
+    const int wg = kFamilyWG[family & 3];
+    const bool gl = transform == kNTT;
+    if (gl && family != 0) return hipErrorInvalidValue;
+    switch (family) {
+        case 1:
+            hipLaunchKernelGGL((ll_batch_kernel<false, 384>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls, d_items,
+                               d_out, d_counters);
+            break;
+        case 2:
+            hipLaunchKernelGGL((ll_batch_kernel<false, 320>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls, d_items,
+                               d_out, d_counters);
+            break;
+        case 3:
+            hipLaunchKernelGGL((ll_batch_kernel<false, 448>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls, d_items,
+                               d_out, d_counters);
+            break;
+        default:
+            if (gl)
+                hipLaunchKernelGGL((ll_batch_kernel<true, 512>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls,
+                                   d_items, d_out, d_counters);
+            else
+                hipLaunchKernelGGL((ll_batch_kernel<false, 512>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls,
+                                   d_items, d_out, d_counters);
+            break;
+    }
+    return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int transform, int family) {
+    const bool gl = transform == kNTT;
+    const int wg = kFamilyWG[family & 3];
     int nb = 0;
-    hipError_t e = transform == kNTT
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<true>, kWG, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false>, kWG, 0);
+    hipError_t e;
+    switch (family) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 384>, wg, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 320>, wg, 0); break;
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 448>, wg, 0); break;
+        default:
+            e = gl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<true, 512>, wg, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 512>, wg, 0);
+            break;
+    }
+    if (gl && family != 0) return 1;
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
 }

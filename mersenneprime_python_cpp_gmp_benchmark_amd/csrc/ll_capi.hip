@@ -30,9 +30,9 @@ hipError_t mlp_forward(const MlpBuffers &B, int n, bool with_backward, hipStream
 hipError_t mlp_gradient(const MlpBuffers &B, int n, hipStream_t st);
 hipError_t mlp_step(const MlpBuffers &B, float lr, int t, hipStream_t st);
 
-hipError_t launch_batch(int transform, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
+hipError_t launch_batch(int transform, int family, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
                         BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream);
-int batch_blocks_per_cu(int transform);
+int batch_blocks_per_cu(int transform, int family);
 
 bool large_launch_for(int transform, u64 M, LargeLaunch &L);
 int large_blocks(const LargeLaunch &L);
@@ -52,10 +52,11 @@ struct TableSet {
 struct DeviceState {
     int id = -1;
     hipStream_t stream = nullptr;
+    hipStream_t fstream[4] = {nullptr, nullptr, nullptr, nullptr};   // batch families (0: stream)
     int cus = 0;
     double wclk_hz = 1e8;
-    int bpc[2] = {0, 0};
-    std::map<std::pair<int, int>, TableSet> tables;
+    int bpc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};                    // workgroups per CU
+    std::map<std::pair<int, int>, TableSet> tables;                   // (transform, class)
     void *d_items = nullptr, *d_out = nullptr, *d_cls = nullptr, *d_cnt = nullptr;
     size_t cap_items = 0, cap_cls = 0;
 };
@@ -74,14 +75,45 @@ static std::vector<double> g_err_trace;     // per-iteration max round-off of th
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
 static std::atomic<double> g_bpw{18.0};        // reference knob: the conservative length rule
-// Bits-per-digit cap of the batched FP64 engine (power-of-two lengths up to 8192).  Its worst
-// round-off over every prime <= 100001 is 0.053 at 20 bits/digit (0.003 at 18), far below the
-// 0.4 rollback threshold, so short lengths take more bits than the reference's 18.
+// Bits-per-digit cap of the batched FP64 engine (lengths up to 8192).  Its worst round-off
+// over every prime <= 100001 is about 0.05 at 20 bits/digit, far below the 0.4 rollback
+// threshold, so short lengths take more bits than the reference's 18.
 static std::atomic<double> g_batch_bpw{20.0};
+// Length families the batched engine may pick (bit f: family f; powers of two always).
+static std::atomic<int> g_batch_families{0xF};
+
+// Measured cost of each batched length: device microseconds per exponent-squaring with the
+// GPU full of exponents of that length (the autotuner's table; ll_batch_costs.inc holds the
+// values measured on an MI355X).  Among the lengths within the bits-per-digit cap the engine
+// runs the cheapest, not merely the shortest.
+struct BatchCost {
+    u64 n;
+    int transform;
+    double us;
+};
+static const BatchCost kBatchCostDefaults[] = {
+#include "ll_batch_costs.inc"
+    {0, 0, 0.0}};
+static std::mutex g_cost_mx;
+static std::map<std::pair<int, u64>, double> &batch_costs() {
+    static std::map<std::pair<int, u64>, double> m = [] {
+        std::map<std::pair<int, u64>, double> d;
+        for (const BatchCost &c : kBatchCostDefaults)
+            if (c.n) d[{c.transform, c.n}] = c.us;
+        return d;
+    }();
+    return m;
+}
+static double batch_cost(int transform, u64 N) {
+    std::lock_guard<std::mutex> lk(g_cost_mx);
+    auto &m = batch_costs();
+    auto it = m.find({transform, N});
+    return it == m.end() ? 0.0 : it->second;
+}
 
 // batched engine limits
-static const int kFp64MinLog = 5, kFp64MaxLog = 13;   // N = 32 .. 8192 (M = 16 .. 4096)
-static const int kNttMinLog = 4, kNttMaxLog = 12;     // N = 16 .. 4096
+static const int kFp64MaxLog = 13;                    // N <= 8192 (M <= 4096)
+static const int kNttMaxLog = 12;                     // N = 16 .. 4096
 
 #define HIPCHK(x)                                                                        \
     do {                                                                                 \
@@ -106,14 +138,18 @@ static int init_locked(int device) {
     }
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    g_dev.fstream[0] = g_dev.stream;
+    for (int f = 1; f < 4; ++f) HIPCHK(hipStreamCreateWithFlags(&g_dev.fstream[f], hipStreamNonBlocking));
     hipDeviceProp_t pr;
     HIPCHK(hipGetDeviceProperties(&pr, device));
     g_dev.cus = pr.multiProcessorCount;
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         g_dev.wclk_hz = (double)khz * 1e3;
-    g_dev.bpc[0] = batch_blocks_per_cu(kFP64);
-    g_dev.bpc[1] = batch_blocks_per_cu(kNTT);
+    for (int f = 0; f < 4; ++f) {
+        g_dev.bpc[0][f] = batch_blocks_per_cu(kFP64, f);
+        g_dev.bpc[1][f] = batch_blocks_per_cu(kNTT, f);
+    }
     g_dev.id = device;
     return LLH_OK;
 }
@@ -150,35 +186,34 @@ static void sincos_frac(u64 j, u64 n, long double &c, long double &s) {
     }
 }
 
-// Tables of one (transform, log_n).  Caller holds g_mx.
-static int get_tables(int transform, int log_n, TableSet &out) {
-    auto key = std::make_pair(transform, log_n);
+// Caller holds g_mx.
+static int ceil_log2(u64 v);
+
+// Tables of batch class c: FP64 (N = 2M): the full w_M table (real-spectrum pairing), the
+// per-pass twiddles and the two-level weights; GL (N = M): full and per-pass roots and the
+// root-of-two weights.
+static int get_tables(int transform, int c, TableSet &out) {
+    auto key = std::make_pair(transform, c);
     auto it = g_dev.tables.find(key);
     if (it != g_dev.tables.end()) {
         out = it->second;
         return LLH_OK;
     }
-    const u64 N = (u64)1 << log_n;
+    const u64 M = batch_class_m(c);
+    std::vector<u32> ex;
+    if (!M || !batch_pass_exponents(c, ex)) {
+        set_error("no batch schedule for this length");
+        return LLH_ERR_UNSUPPORTED;
+    }
     TableSet ts;
     if (transform == kFP64) {
-        const u64 M = N / 2;
-        std::vector<double> tw(2 * M), wf(N), wi(N);
+        const u64 N = 2 * M;
+        std::vector<double> tw(2 * M);
         for (u64 j = 0; j < M; ++j) {
-            long double c, s;
-            sincos_frac(j, M, c, s);
-            tw[2 * j] = (double)c;
-            tw[2 * j + 1] = (double)(-s);          // w_M^j = exp(-2 pi i j / M)
-        }
-        for (u64 e = 0; e < N; ++e) {
-            long double w = exp2l((long double)e / (long double)N);
-            wf[e] = (double)w;
-            wi[e] = (double)(1.0L / (w * (long double)M));
-        }
-        // per-pass twiddle order of the class's schedule
-        std::vector<u32> ex;
-        if (!batch_pass_exponents(log_n - 1, ex)) {
-            set_error("no batch schedule for this length");
-            return LLH_ERR_UNSUPPORTED;
+            long double cs, sn;
+            sincos_frac(j, M, cs, sn);
+            tw[2 * j] = (double)cs;
+            tw[2 * j + 1] = (double)(-sn);          // w_M^j = exp(-2 pi i j / M)
         }
         std::vector<double> twp(2 * ex.size() + 2);
         for (size_t e = 0; e < ex.size(); ++e) {
@@ -186,8 +221,8 @@ static int get_tables(int transform, int log_n, TableSet &out) {
             twp[2 * e + 1] = tw[2 * ex[e] + 1];
         }
         // two-level weights: e = lo + S hi, w(e) = 2^(lo/N) 2^(S hi/N) (inverse with 1/M)
-        const int log_s = (log_n + 1) / 2;
-        const u64 S = (u64)1 << log_s, HS = N >> log_s;
+        const int log_s = (ceil_log2(N) + 1) / 2;
+        const u64 S = (u64)1 << log_s, HS = (N + S - 1) >> log_s;
         if (S > (u64)kWtabSlot || HS > (u64)kWtabSlot) {
             set_error("weight table slot too small");
             return LLH_ERR_UNSUPPORTED;
@@ -203,32 +238,24 @@ static int get_tables(int transform, int log_n, TableSet &out) {
         }
         ts.log_s = log_s;
         HIPCHK(hipMalloc(&ts.tw, tw.size() * 8));
-        HIPCHK(hipMalloc(&ts.wf, N * 8));
-        HIPCHK(hipMalloc(&ts.wi, N * 8));
         HIPCHK(hipMalloc(&ts.twp, twp.size() * 8));
         HIPCHK(hipMalloc(&ts.wtab, wtab.size() * 8));
         HIPCHK(hipMemcpy(ts.tw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ts.wf, wf.data(), N * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ts.wi, wi.data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ts.twp, twp.data(), twp.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ts.wtab, wtab.data(), wtab.size() * 8, hipMemcpyHostToDevice));
     } else {
+        const u64 N = M;
         const u64 w = gl_root_unity_aligned(N), wv = gl_inv_h(w);
         const u64 r2 = gl_root2_h(N), r2i = gl_inv_h(r2), ninv = gl_inv_h(N);
-        if (!w || !r2) {
+        if (!batch_class_pow2(c) || !w || !r2) {
             set_error("no Goldilocks roots for this length");
             return LLH_ERR_UNSUPPORTED;
         }
         std::vector<u64> tw(N), twi(N), wf(N), wi(N);
-        u64 a = 1, b = 1, c = 1, d = ninv;
+        u64 a = 1, b = 1, cc = 1, d = ninv;
         for (u64 j = 0; j < N; ++j) {
-            tw[j] = a; twi[j] = b; wf[j] = c; wi[j] = d;
-            a = gl_mul_h(a, w); b = gl_mul_h(b, wv); c = gl_mul_h(c, r2); d = gl_mul_h(d, r2i);
-        }
-        std::vector<u32> ex;
-        if (!batch_pass_exponents(log_n, ex)) {
-            set_error("no batch schedule for this length");
-            return LLH_ERR_UNSUPPORTED;
+            tw[j] = a; twi[j] = b; wf[j] = cc; wi[j] = d;
+            a = gl_mul_h(a, w); b = gl_mul_h(b, wv); cc = gl_mul_h(cc, r2); d = gl_mul_h(d, r2i);
         }
         std::vector<u64> twp(ex.size() + 1), twpi(ex.size() + 1);
         for (size_t e = 0; e < ex.size(); ++e) {
@@ -253,29 +280,75 @@ static int get_tables(int transform, int log_n, TableSet &out) {
     return LLH_OK;
 }
 
-// log2 of the batched engine's length for p (0 = not in the batched engine).
-static int batch_log_n(u64 p, int transform) {
-    if (p < 64) return 0;
-    // the kernels index digits with 32-bit products j * p, j < N
-    auto fits = [p](int l) { return (((u64)1 << l) + 2) * p < ((u64)1 << 32); };
-    if (transform == kFP64) {
-        const double bpw = g_batch_bpw.load();
-        for (int l = kFp64MinLog; l <= kFp64MaxLog; ++l)
-            if ((double)p / (double)((u64)1 << l) <= bpw) return fits(l) ? l : 0;
-        return 0;
-    }
-    for (int l = kNttMinLog; l <= kNttMaxLog; ++l) {
-        const u64 N = (u64)1 << l;
-        if ((p + N - 1) / N <= (u64)ntt_max_bits(N)) return fits(l) ? l : 0;
-    }
+// Batched-engine lengths (N, class), ascending: FP64 N = 2M over every class, GL N = M over
+// the power-of-two classes.
+static const std::vector<std::pair<u64, int>> &batch_ladder(int transform) {
+    static const std::vector<std::pair<u64, int>> lad[2] = {
+        [] {
+            std::vector<std::pair<u64, int>> v;
+            for (int c = 1; c < batch_num_classes(); ++c) v.push_back({2 * batch_class_m(c), c});
+            std::sort(v.begin(), v.end());
+            return v;
+        }(),
+        [] {
+            std::vector<std::pair<u64, int>> v;
+            for (int c = 1; c < batch_num_classes(); ++c)
+                if (batch_class_pow2(c)) v.push_back({batch_class_m(c), c});
+            std::sort(v.begin(), v.end());
+            return v;
+        }()};
+    return lad[transform == kNTT ? 1 : 0];
+}
+
+// the kernels index digits with 32-bit products j * p, j < N
+static bool batch_fits(u64 p, u64 N) { return (N + 2) * p < ((u64)1 << 32); }
+
+// Batch class of length N (0 = not a batched length).
+static int batch_class_of(int transform, u64 N) {
+    for (const auto &e : batch_ladder(transform))
+        if (e.first == N) return e.second;
     return 0;
 }
 
-static int groups_per_wg(int log_m) {
-    // 512-thread workgroups; G = M / R threads per exponent, R = 8 (R = 4 for M < 64)
-    const int M = 1 << log_m;
-    const int R = M >= 64 ? 8 : 4;
-    return 512 / (M / R);
+// The batched engine's length for p (0 = not in the batched engine).  Candidates: FP64 the
+// lengths with p / N <= the batch bits-per-digit cap, GL those within the exactness bound, in
+// the enabled families, up to twice the shortest.  Of these the one with the lowest measured
+// cost runs (a length without a measured cost never displaces a shorter one).
+static u64 batch_length(u64 p, int transform) {
+    if (p < 64) return 0;
+    const double bpw = g_batch_bpw.load();
+    const int fam = g_batch_families.load() | 1;
+    u64 best = 0, shortest = 0;
+    double best_cost = 0.0;
+    for (const auto &e : batch_ladder(transform)) {
+        const u64 N = e.first;
+        if (!((fam >> batch_class_family(e.second)) & 1)) continue;
+        const bool ok = transform == kFP64 ? (double)p / (double)N <= bpw
+                                           : (p + N - 1) / N <= (u64)ntt_max_bits(N);
+        if (!ok) continue;
+        if (!shortest) {
+            if (!batch_fits(p, N)) return 0;
+            shortest = N;
+        }
+        if (N > 2 * shortest || !batch_fits(p, N)) break;
+        const double c = batch_cost(transform, N);
+        if (!best) {
+            best = N;
+            best_cost = c;
+        } else if (c > 0.0 && best_cost > 0.0 && c < best_cost) {
+            best = N;
+            best_cost = c;
+        }
+    }
+    return best;
+}
+
+// next batched length above N for p (0 = none): the rollback target
+static u64 next_batch_length(u64 p, u64 N, int transform) {
+    const int fam = g_batch_families.load() | 1;
+    for (const auto &e : batch_ladder(transform))
+        if (e.first > N && ((fam >> batch_class_family(e.second)) & 1)) return batch_fits(p, e.first) ? e.first : 0;
+    return 0;
 }
 
 static int grow(void **ptr, size_t &cap, size_t need) {
@@ -292,51 +365,64 @@ struct Pending {
     u64 p;
     u64 iters;
     size_t idx;
-    int log_n;          // batched engine: log2 of the length
-    u64 n = 0;          // large engine: the length
+    int cls;            // batched engine: size class (0 = the one-word lane, p < 64)
+    u64 n = 0;          // the length (batched or large engine)
 };
 
-// One persistent launch over `work`; fills res[i] for work[i].
+// The persistent launches over `work` (one per class family, on separate streams so each
+// family's tail is back-filled by the others); fills res[i] for work[i].
 static int run_launch(int transform, std::vector<Pending> &work, std::vector<BatchResult> &res) {
     const size_t n = work.size();
     res.assign(n, BatchResult());
     if (!n) return LLH_OK;
+    // family launch order: odd-factor families (the longest exponents) first, powers of two and
+    // the one-word lane last
+    auto fam_rank = [](int c) {
+        const int f = batch_class_family(c);
+        return f == 0 ? 3 : f - 1;
+    };
     std::vector<size_t> order(n);
     for (size_t i = 0; i < n; ++i) order[i] = i;
-    // longest first: by length descending, p descending; the one-word lane (log_n 0) last
+    // per family: longest first, by length descending, p descending; the one-word lane last
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-        if (work[a].log_n != work[b].log_n) return work[a].log_n > work[b].log_n;
-        return work[a].p > work[b].p;
+        const Pending &x = work[a], &y = work[b];
+        const int fx = fam_rank(x.cls), fy = fam_rank(y.cls);
+        if (fx != fy) return fx < fy;
+        const u64 nx = x.cls ? x.n : 0, ny = y.cls ? y.n : 0;
+        if (nx != ny) return nx > ny;
+        return x.p > y.p;
     });
     std::vector<BatchItem> items(n);
     std::vector<ClassDesc> cls;
-    size_t waves = 0;
+    std::vector<int> fam_of;            // family of each class entry
     for (size_t k = 0; k < n; ++k) {
         const Pending &w = work[order[k]];
         items[k].p = (u32)w.p;
-        items[k].log_n = (u32)w.log_n;
+        items[k].log_n = 0;
         items[k].iters = w.iters;
         items[k].slot = (u32)order[k];
         items[k].pad = 0;
-        if (cls.empty() || cls.back().log_n != w.log_n) {
+        if (cls.empty() || cls.back().cls != w.cls) {
             ClassDesc cd;
             std::memset(&cd, 0, sizeof(cd));
-            cd.log_n = w.log_n;
-            cd.log_m = w.log_n == 0 ? 0 : (transform == kFP64 ? w.log_n - 1 : w.log_n);
+            cd.cls = w.cls;
+            const u64 M = batch_class_m(w.cls);
+            cd.log_m = M ? ceil_log2(M) : 0;
+            cd.log_n = w.cls ? ceil_log2(w.n) : 0;     // exact for the power-of-two classes
             cd.first = (u32)k;
             cd.count = 0;
-            if (w.log_n) {
+            if (w.cls) {
                 TableSet ts;
-                int rc = get_tables(transform, w.log_n, ts);
+                int rc = get_tables(transform, w.cls, ts);
                 if (rc) return rc;
                 cd.tw = ts.tw; cd.twi = ts.twi; cd.wf = ts.wf; cd.wi = ts.wi;
                 cd.twp = ts.twp; cd.twpi = ts.twpi; cd.wtab = ts.wtab; cd.log_s = ts.log_s;
             }
             cls.push_back(cd);
+            fam_of.push_back(batch_class_family(w.cls));
         }
         cls.back().count++;
     }
-    for (auto &c : cls) waves += c.log_m ? (c.count + groups_per_wg(c.log_m) - 1) / groups_per_wg(c.log_m) : 1;
     int rc;
     if ((rc = grow(&g_dev.d_items, g_dev.cap_items, n * sizeof(BatchItem) + n * sizeof(BatchResult) + 64)))
         return rc;
@@ -350,12 +436,34 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
     HIPCHK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(ClassDesc), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(d_cnt, 0, cls.size() * sizeof(u32), st));
     const int tr = transform == kNTT ? 1 : 0;
-    int grid = (int)std::min<size_t>(std::max<size_t>(waves, 1), (size_t)g_dev.cus * g_dev.bpc[tr]);
-    hipEvent_t ev0, ev1;
+    hipEvent_t ev0, ev1, evf[4];
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    for (int f = 0; f < 4; ++f) HIPCHK(hipEventCreateWithFlags(&evf[f], hipEventDisableTiming));
     HIPCHK(hipEventRecord(ev0, st));
-    HIPCHK(launch_batch(transform, d_cls, (int)cls.size(), d_items, d_out, d_cnt, grid, st));
+    bool joined[4] = {false, false, false, false};
+    for (size_t i = 0; i < cls.size();) {
+        // the run of class entries of one family
+        size_t j = i;
+        size_t waves = 0;
+        while (j < cls.size() && fam_of[j] == fam_of[i]) {
+            const int g = batch_class_groups(cls[j].cls);
+            waves += cls[j].cls ? (cls[j].count + g - 1) / g : 1;
+            ++j;
+        }
+        const int f = fam_of[i];
+        hipStream_t fs = g_dev.fstream[f];
+        if (fs != st) HIPCHK(hipStreamWaitEvent(fs, ev0, 0));
+        const int grid = (int)std::min<size_t>(std::max<size_t>(waves, 1), (size_t)g_dev.cus * g_dev.bpc[tr][f]);
+        HIPCHK(launch_batch(transform, f, d_cls + i, (int)(j - i), d_items, d_out, d_cnt + i, grid, fs));
+        if (fs != st) {
+            HIPCHK(hipEventRecord(evf[f], fs));
+            joined[f] = true;
+        }
+        i = j;
+    }
+    for (int f = 0; f < 4; ++f)
+        if (joined[f]) HIPCHK(hipStreamWaitEvent(st, evf[f], 0));
     HIPCHK(hipEventRecord(ev1, st));
     HIPCHK(hipMemcpyAsync(res.data(), d_out, n * sizeof(BatchResult), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -363,6 +471,7 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
+    for (int f = 0; f < 4; ++f) (void)hipEventDestroy(evf[f]);
     g_last_kernel_ms += ms;
     g_last_launches += 1;
     return LLH_OK;
@@ -673,22 +782,24 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
         if (p % 2 == 0) continue;                              // :32
         if (p >= ((u64)1 << 31)) { status[i] = kStNotRun; continue; }
         const u64 it = max_iters ? std::min<u64>(max_iters, p - 2) : p - 2;
-        const int batch_max = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
-        const int batch_min = transform == kFP64 ? kFp64MinLog : kNttMinLog;
-        if (force_n && p >= 64) {
-            const int fl = (force_n & (force_n - 1)) ? 0 : ceil_log2(force_n);
-            if (fl >= batch_min && fl <= batch_max && (force_n + 2) * p < ((u64)1 << 32))
-                work.push_back(Pending{p, it, i, fl});
+        if (p < 64) {                                          // the one-word lane
+            work.push_back(Pending{p, it, i, 0, 1});
+            continue;
+        }
+        if (force_n) {
+            const int c = batch_class_of(transform, force_n);
+            if (c && batch_fits(p, force_n))
+                work.push_back(Pending{p, it, i, c, force_n});
             else
-                large.push_back(Pending{p, it, i, 0, force_n});
+                large.push_back(Pending{p, it, i, -1, force_n});
             continue;
         }
-        const int ln = batch_log_n(p, transform);
-        if (p >= 64 && ln == 0) {
-            large.push_back(Pending{p, it, i, 0, large_length(p, transform)});
+        const u64 bn = batch_length(p, transform);
+        if (bn == 0) {
+            large.push_back(Pending{p, it, i, -1, large_length(p, transform)});
             continue;
         }
-        work.push_back(Pending{p, it, i, ln});
+        work.push_back(Pending{p, it, i, batch_class_of(transform, bn), bn});
     }
     // exponents beyond the batched engine: one at a time on the large engine (longest first)
     for (const Pending &w : large) {
@@ -711,7 +822,6 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
         status[w.idx] = r.status;
         if (n_used) n_used[w.idx] = (uint32_t)N;
     }
-    const int max_log = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
     while (!work.empty()) {
         std::vector<BatchResult> res;
         rc = run_launch(transform, work, res);
@@ -722,15 +832,17 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
             const BatchResult &r = res[k];
             secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
             errs[w.idx] = std::max(errs[w.idx], r.maxerr);
-            if (r.status == kStRoundoff && w.log_n < max_log && !no_rollback &&
-                (((u64)2 << w.log_n) + 2) * w.p < ((u64)1 << 32)) {
-                again.push_back(Pending{w.p, w.iters, w.idx, w.log_n + 1});   // rollback at 2N
-                continue;
+            if (r.status == kStRoundoff && w.cls > 0 && !no_rollback) {
+                const u64 nx = next_batch_length(w.p, w.n, transform);   // rollback: redo longer
+                if (nx) {
+                    again.push_back(Pending{w.p, w.iters, w.idx, batch_class_of(transform, nx), nx});
+                    continue;
+                }
             }
             verdict[w.idx] = r.verdict;
             res64[w.idx] = r.res64;
             status[w.idx] = r.status;
-            if (n_used) n_used[w.idx] = w.log_n ? (1u << w.log_n) : 1u;
+            if (n_used) n_used[w.idx] = (uint32_t)w.n;
         }
         work.swap(again);
     }
@@ -780,8 +892,7 @@ uint64_t llh_fft_length(uint64_t p, double max_bpw) {
 
 uint64_t llh_batch_length(uint64_t p, int transform) {
     if (p < 64) return p >= 3 ? 1 : 0;
-    int l = batch_log_n(p, transform);
-    return l ? ((uint64_t)1 << l) : 0;
+    return batch_length(p, transform);
 }
 
 uint64_t llh_ntt_length(uint64_t p) { return ntt_length_ref(p); }
@@ -789,9 +900,9 @@ uint64_t llh_ntt_length(uint64_t p) { return ntt_length_ref(p); }
 uint64_t llh_fp64_length(uint64_t p, double max_bpw) {
     if (p < 64) return 0;
     if (!(max_bpw > 0)) max_bpw = g_bpw.load();
-    for (int l = kFp64MinLog; l <= kFp64MaxLog; ++l) {
-        const u64 N = (u64)1 << l;
-        if ((double)p / (double)N <= max_bpw && (N + 2) * p < ((u64)1 << 32)) return N;
+    for (const auto &e : batch_ladder(kFP64)) {
+        const u64 N = e.first;
+        if ((double)p / (double)N <= max_bpw && batch_fits(p, N)) return N;
     }
     for (u64 N : large_lengths(kFP64))
         if ((double)p / (double)N <= max_bpw) return N;
@@ -824,6 +935,46 @@ int llh_set_batch_bits_per_word(double bits) {
 }
 
 double llh_get_batch_bits_per_word(void) { return g_batch_bpw.load(); }
+
+int llh_set_batch_families(int mask) {
+    if (mask & ~0xF) {
+        set_error("family mask has bits 0..3 only");
+        return LLH_ERR_ARG;
+    }
+    g_batch_families.store(mask | 1);
+    return LLH_OK;
+}
+
+int llh_get_batch_families(void) { return g_batch_families.load(); }
+
+int llh_set_batch_cost(uint64_t n, int transform, double us) {
+    if ((transform != kFP64 && transform != kNTT) || !batch_class_of(transform, n) || !(us >= 0)) {
+        set_error("llh_set_batch_cost: not a batched length, or a negative cost");
+        return LLH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_cost_mx);
+    if (us == 0.0) batch_costs().erase({transform, n});
+    else batch_costs()[{transform, n}] = us;
+    return LLH_OK;
+}
+
+double llh_get_batch_cost(uint64_t n, int transform) { return batch_cost(transform, n); }
+
+int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, uint64_t n_force, int flags,
+                int32_t *verdict, uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used,
+                int32_t *status) {
+    if (n_force && (n_force < 16 || n_force > ((uint64_t)1 << 24))) {
+        set_error("n_force must be in [16, 2^24]");
+        return LLH_ERR_ARG;
+    }
+    try {
+        return batch_impl(ps, n, max_iters, transform, verdict, res64, seconds, maxerr, n_used, status, n_force,
+                          (flags & 1) != 0);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return LLH_ERR_HIP;
+    }
+}
 
 int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, int32_t *verdict, uint64_t *res64,
               double *seconds, double *maxerr, uint32_t *n_used, int32_t *status) {
@@ -865,9 +1016,7 @@ int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n
 
 size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap) {
     std::vector<u64> v;
-    const int lo = transform == kFP64 ? kFp64MinLog : kNttMinLog;
-    const int hi = transform == kFP64 ? kFp64MaxLog : kNttMaxLog;
-    for (int l = lo; l <= hi; ++l) v.push_back((u64)1 << l);
+    for (const auto &e : batch_ladder(transform)) v.push_back(e.first);
     for (u64 n : large_lengths(transform))
         if (std::find(v.begin(), v.end(), n) == v.end()) v.push_back(n);
     std::sort(v.begin(), v.end());

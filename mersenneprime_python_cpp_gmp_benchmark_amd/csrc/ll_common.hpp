@@ -24,6 +24,26 @@ typedef int32_t i32;
 #define LL_DEV __device__ __forceinline__
 #define LL_HD __host__ __device__ __forceinline__
 
+// Digit layout of a length-N IBDWT of 2^p - 1, for any N: digit d has b_d = c_{d+1} - c_d
+// bits, c_d = ceil(d p / N), and weight exponent e_d = c_d N - d p.  Both follow from the
+// residue r_d = d p mod N alone: with P = p mod N and F = floor(p / N),
+//   b_d = F + [r_d + P >= N] + [r_{d+1} > 0] - [r_d > 0],   e_d = (N - r_d) mod N,
+//   r_{d+1} = (r_d + P) mod N,   r_{d+D} = (r_d + (D p mod N)) mod N,
+// so walking digits needs no division.
+struct DigitWalk {
+    u32 n, P, F;
+    LL_HD u32 add(u32 r, u32 delta) const {          // (r + delta) mod N, r, delta < N
+        const u32 v = r + delta;
+        return v >= n ? v - n : v;
+    }
+    LL_HD u32 next(u32 r) const { return add(r, P); }
+    LL_HD int bits(u32 r) const {
+        const u32 rn = next(r);
+        return (int)(F + (r + P >= n ? 1u : 0u) + (rn > 0 ? 1u : 0u)) - (r > 0 ? 1 : 0);
+    }
+    LL_HD u32 wexp(u32 r) const { return r ? n - r : 0u; }
+};
+
 // Returns v unchanged, but opaque to the optimiser.  Index math derived from it is then
 // recomputed where it is used instead of being hoisted out of the squaring loop and kept
 // live in registers for its whole length (twiddle and weight addresses are loop-invariant,

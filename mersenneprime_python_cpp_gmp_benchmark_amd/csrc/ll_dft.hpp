@@ -320,17 +320,28 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
     constexpr int NBF = H / r;
     constexpr int NB = (NBF + G - 1) / G;
     constexpr int TSTEP = H / (r * L);
+    constexpr int P = 1 << PS;
     static_assert(H % (r * L) == 0, "radix schedule does not divide the length");
+    // Fast addressing as in stockham_pass: with G a multiple of L, k = (t + G b) mod L = t mod L;
+    // with G, NBF and L multiples of P every padded index is a per-thread base plus a constant.
+    constexpr bool FAST = G % L == 0 && G % P == 0 && NBF % P == 0 && (L == 1 ? r == P : L % P == 0);
     T y[NB][r];
     t = ll_opaque(t);
+    const int kt = t % L;
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int i = t + G * b;
-        if (i < NBF) {
-            const int k = i % L;
+        if (NBF % G == 0 || i < NBF) {
+            const int k = FAST ? kt : i % L;
+            if constexpr (FAST) {
+                const int rb = lds_pad_s<PS>(t);
 #pragma unroll
-            for (int s = 0; s < r; ++s) y[b][s] = lds[lds_pad_s<PS>(i + s * NBF)];
+                for (int s = 0; s < r; ++s) y[b][s] = lds[rb + (G * b + s * NBF) / P * (P + 1)];
+            } else {
+#pragma unroll
+                for (int s = 0; s < r; ++s) y[b][s] = lds[lds_pad_s<PS>(i + s * NBF)];
+            }
             if constexpr (L > 1) {
 #pragma unroll
                 for (int s = 1; s < r; ++s) {
@@ -342,13 +353,23 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
         }
     }
     __syncthreads();
+    // output q of butterfly i goes to (i - k) r + k + q L
+    const int wb = L == 1 ? t * (r + 1) : (t - kt) * r / P * (P + 1) + kt + (kt >> PS);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int i = t + G * b;
-        if (i < NBF) {
-            const int k = i % L;
+        if (NBF % G == 0 || i < NBF) {
+            if constexpr (FAST) {
 #pragma unroll
-            for (int q = 0; q < r; ++q) lds[lds_pad_s<PS>((i - k) * r + k + q * L)] = y[b][q];
+                for (int q = 0; q < r; ++q) {
+                    const int off = L == 1 ? G * b * (r + 1) + q : (G * b * r + q * L) / P * (P + 1);
+                    lds[wb + off] = y[b][q];
+                }
+            } else {
+                const int k = i % L;
+#pragma unroll
+                for (int q = 0; q < r; ++q) lds[lds_pad_s<PS>((i - k) * r + k + q * L)] = y[b][q];
+            }
         }
     }
 }

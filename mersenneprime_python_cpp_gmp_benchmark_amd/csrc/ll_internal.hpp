@@ -57,12 +57,19 @@ struct ClassDesc {
     const void *wtab;   // FP64: two-level weights, 4 x kWtabSlot doubles: forward low | forward
                         // high | inverse low | inverse high, w(e) = low[e & (S-1)] * high[e >> log_s]
     i32 log_s;          // S = 2^log_s
-    i32 pad;
+    i32 cls;            // size class (ll_batch.hip); 0 = the one-word lane
 };
 constexpr int kWtabSlot = 128;
 
-// Batched engine: radix schedule of a class and the per-pass twiddle order (ll_batch.hip).
-bool batch_pass_exponents(int log_m, std::vector<u32> &ex);
+// ---- batched engine size classes and launches (ll_batch.hip) ------------------------------
+int batch_num_classes();
+int batch_class_of_m(u64 M);            // class of transform length M (0 = none)
+u64 batch_class_m(int c);               // transform length of class c
+bool batch_class_pow2(int c);
+int batch_class_family(int c);          // 0: powers of two, 1: 3 * 2^k, 2: 5 * 2^k, 3: 7 * 2^k
+int batch_class_groups(int c);          // exponents per workgroup
+int batch_family_wg(int family);
+bool batch_pass_exponents(int c, std::vector<u32> &ex);   // per-pass twiddle order of class c
 
 // ---- large engine (ll_large.hip) ----------------------------------------------------------
 // Kernel arguments (by value).  Table pointers are device addresses.

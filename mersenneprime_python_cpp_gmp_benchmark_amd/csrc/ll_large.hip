@@ -158,22 +158,6 @@ LL_DEV u32 wexp(u64 d, u32 p, const NDiv &D) {
 }
 LL_HD NDiv ndiv_of(const LargeArgs &A) { return NDiv{A.n, A.inv_n, A.log_n}; }
 
-// Walks digit positions by the residue r_d = d p mod N alone: with P = p mod N, F = p div N,
-//   b_d = F + [r_d + P >= N] + [r_{d+1} > 0] - [r_d > 0],   e_d = (N - r_d) mod N,
-//   r_{d+1} = (r_d + P) mod N,   r_{d+D} = (r_d + (D p mod N)) mod N.
-struct DigitWalk {
-    u32 n, P, F;
-    LL_DEV u32 add(u32 r, u32 delta) const {          // (r + delta) mod N, r, delta < N
-        const u32 v = r + delta;
-        return v >= n ? v - n : v;
-    }
-    LL_DEV u32 next(u32 r) const { return add(r, P); }
-    LL_DEV int bits(u32 r) const {
-        const u32 rn = next(r);
-        return (int)(F + (r + P >= n ? 1u : 0u) + (rn > 0 ? 1u : 0u)) - (r > 0 ? 1 : 0);
-    }
-    LL_DEV u32 wexp(u32 r) const { return r ? n - r : 0u; }
-};
 LL_DEV DigitWalk walk_of(const LargeArgs &A) { return DigitWalk{(u32)A.n, A.p_mod_n, A.p_div_n}; }
 // d p mod N for one position (one exact division; used once per thread or per row)
 LL_DEV u32 rmod(u64 d, u32 p, const NDiv &D) {

@@ -53,6 +53,9 @@ def lib():
         L.llh_get_fft_bits_per_word.restype = dbl
         L.llh_set_batch_bits_per_word.argtypes = [dbl]
         L.llh_get_batch_bits_per_word.restype = dbl
+        L.llh_get_batch_cost.restype = dbl
+        L.llh_get_batch_cost.argtypes = [u64, ctypes.c_int]
+        L.llh_set_batch_cost.argtypes = [u64, ctypes.c_int, dbl]
         L.llh_batch.argtypes = [P(u64), ctypes.c_size_t, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(dbl),
                                 P(u32), P(i32)]
         L.llh_last_launch_stats.argtypes = [P(dbl), P(u32)]
@@ -102,6 +105,15 @@ def get_batch_bpw() -> float:
     return lib().llh_get_batch_bits_per_word()
 
 
+def set_batch_families(mask: int) -> None:
+    """Length families of the batched FP64 engine: bit 1 = 3*2^k, bit 2 = 5*2^k, bit 3 = 7*2^k."""
+    _check(lib().llh_set_batch_families(mask))
+
+
+def get_batch_families() -> int:
+    return lib().llh_get_batch_families()
+
+
 def batch(ps, iters: int = 0, transform: int = NTT) -> dict:
     """Run llh_batch; returns numpy arrays verdict, res64, seconds, maxerr, n_used, status."""
     ps = np.ascontiguousarray(np.asarray(ps, dtype=np.uint64))
@@ -117,6 +129,31 @@ def batch(ps, iters: int = 0, transform: int = NTT) -> dict:
                            c(out["maxerr"], ctypes.c_double), c(out["n_used"], ctypes.c_uint32),
                            c(out["status"], ctypes.c_int32)))
     return out
+
+
+def batch_n(ps, iters: int = 0, transform: int = FP64, n: int = 0, no_rollback: bool = False) -> dict:
+    """llh_batch_n: every exponent at transform length n (0 = the engine's choice)."""
+    ps = np.ascontiguousarray(np.asarray(ps, dtype=np.uint64))
+    k = len(ps)
+    out = {
+        "verdict": np.zeros(k, np.int32), "res64": np.zeros(k, np.uint64), "seconds": np.zeros(k, np.float64),
+        "maxerr": np.zeros(k, np.float64), "n_used": np.zeros(k, np.uint32), "status": np.zeros(k, np.int32),
+    }
+    P = ctypes.POINTER
+    c = lambda a, t: a.ctypes.data_as(P(t))  # noqa: E731
+    _check(lib().llh_batch_n(c(ps, ctypes.c_uint64), k, iters, transform, n, 1 if no_rollback else 0,
+                             c(out["verdict"], ctypes.c_int32), c(out["res64"], ctypes.c_uint64),
+                             c(out["seconds"], ctypes.c_double), c(out["maxerr"], ctypes.c_double),
+                             c(out["n_used"], ctypes.c_uint32), c(out["status"], ctypes.c_int32)))
+    return out
+
+
+def set_batch_cost(n: int, transform: int, us: float) -> None:
+    _check(lib().llh_set_batch_cost(n, transform, us))
+
+
+def get_batch_cost(n: int, transform: int = FP64) -> float:
+    return lib().llh_get_batch_cost(n, transform)
 
 
 def lucas_lehmer(p: int, iters: int = 0, transform: int = NTT) -> dict:

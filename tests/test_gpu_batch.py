@@ -84,3 +84,61 @@ def test_forced_rollback_stays_exact():
     _check_rows(sel, out)
     assert (out["n_used"] > first).any()          # some exponents were rolled back to 2N
     assert (out["n_used"] >= first).all()
+
+
+def _fp64_ladder():
+    lens = llhip.supported_lengths(llhip.FP64)
+    return [n for n in lens if n <= 8192]
+
+
+def test_every_batch_length_matches_gmp():
+    """Every FP64 batch length (powers of two and 3, 5, 7 times a power of two), forced onto the
+    three largest primes within 20 bits per digit, in one batch per length: Res64 after 1500
+    squarings (and, for the shorter lengths, after the full test) must equal GMP's."""
+    from oracle import oracle as O
+    lad = _fp64_ladder()
+    assert {384, 640, 896, 1536, 2560, 3584, 5120, 6144, 7168} <= set(lad)
+    for n in lad:
+        if n < 64:
+            continue
+        ps = [p for p in O.primes(max(65, int(n * 14)), int(n * 20)) if (n + 2) * p < (1 << 32)][-3:]
+        out = llhip.batch_n(ps, 1500, llhip.FP64, n, no_rollback=True)
+        assert (out["status"] == llhip.ST_OK).all(), (n, ps, list(out["status"]))
+        assert (out["n_used"] == n).all(), n
+        for i, p in enumerate(ps):
+            assert int(out["res64"][i]) == O.ll_gmp(p, 1500)[1], (p, n)
+        if ps[-1] < 12000:
+            full = llhip.batch_n(ps, 0, llhip.FP64, n, no_rollback=True)
+            for i, p in enumerate(ps):
+                v, r, _ = O.ll_gmp(p)
+                assert (bool(full["verdict"][i]), int(full["res64"][i])) == (v, r), (p, n)
+
+
+def test_length_choice_is_cheapest_within_cap():
+    """The engine's length for p is, among the lengths within the bits-per-digit cap (up to
+    twice the shortest), the one with the lowest measured cost."""
+    L = llhip.lib()
+    cap = llhip.get_batch_bpw()
+    lad = _fp64_ladder()
+    for p in (9689, 11213, 21701, 44497, 60013, 86243, 99991):
+        n = L.llh_batch_length(p, llhip.FP64)
+        ok = [m for m in lad if p / m <= cap]
+        cands = [m for m in ok if m <= 2 * ok[0]]
+        costs = {m: llhip.get_batch_cost(m, llhip.FP64) for m in cands}
+        assert n in cands, (p, n)
+        assert costs[n] == min(costs.values()), (p, n, costs)
+
+
+def test_odd_lengths_forced_match_gmp():
+    """One exponent forced onto every batch length between 6 and 20 bits per digit gives one
+    Res64 (below about 4 bits per digit the two-round parallel carry no longer balances the
+    digits and the round-off check rolls the exponent back)."""
+    from oracle import oracle as O
+    for p, k in ((24989, 2000), (99991, 1000)):
+        want = O.ll_gmp(p, k)[1]
+        for n in _fp64_ladder():
+            if not 6.0 <= p / n <= 20.0:
+                continue
+            r = llhip.lucas_lehmer_n(p, k, llhip.FP64, n)
+            assert r["status"] == llhip.ST_OK and r["n_used"] == n, (p, n, r)
+            assert r["res64"] == want, (p, n)
