@@ -215,12 +215,14 @@ def main() -> int:
     achieved = flops / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_ll_batch_kernel_fp64_p100001.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and tr == llhip.FP64:
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "kernel": "ll_batch_kernel<false> (FP64)" if tr == llhip.FP64 else "ll_batch_kernel<true> (NTT)",
+                "kernel": ("ll_batch_kernel<false, WG> (FP64; one persistent kernel per length family, run "
+                           "concurrently; timed over the whole fork-join)") if tr == llhip.FP64
+                else "ll_batch_kernel<true, 512> (NTT)",
                 "kernel_ms_per_launch": round(per_launch_s * 1e3, 3),
                 "flops_per_launch": flops,
                 "note": "FP64 vector-ALU bound (no MFMA on this path); flops counted at the FP64 length "
@@ -253,6 +255,30 @@ def main() -> int:
                                "res64_matches_gmp": interim.get((P24, k24)) == ntt["res64"]
                                if (P24, k24) in interim else None}
         extra["p16777213_5000_iters"] = ent
+        # learned vs plain length on the held-out exponents of the round-off model (primes drawn
+        # with seed 0 from [2^20, 2^25)) where the model picks another length than the rule
+        try:
+            from mersenneprime_python_cpp_gmp_benchmark_amd import nselect
+            from mersenneprime_python_cpp_gmp_benchmark_amd.roundoff_data import exponents
+            model = nselect.load_model()
+            k4 = 5000
+            diff = [(p, nselect.conservative_length(p), nselect.choose_length(p, model)) for p in exponents("heldout")]
+            diff = [d for d in diff if d[1] != d[2]]
+            tp = tl = 0.0
+            eq = 0
+            for p, nc, nl in diff:
+                rc = run_single(llhip, p, k4, llhip.FP64, nc)
+                rl = run_single(llhip, p, k4, llhip.FP64, nl)
+                tp += rc["wall"]
+                tl += rl["wall"]
+                eq += rc["res64"] == rl["res64"]
+            extra["c4_heldout_learned_vs_plain"] = {
+                "iters": k4, "heldout": len(exponents("heldout")), "learned_differs": len(diff),
+                "seconds_plain": round(tp, 3), "seconds_learned": round(tl, 3),
+                "speedup_vs_plain": round(tp / tl, 3) if tl else None, "res64_equal": eq,
+                "lengths": [[p, nc, nl] for p, nc, nl in diff]}
+        except Exception as e:  # noqa: BLE001 -- the model or its data may be absent
+            log(f"held-out comparison skipped: {e}")
         # plain GPU form of the batch: one exponent per launch, a sample across the range
         sample = ps[len(ps) // 16::len(ps) // 8][:8]
         t1 = time.perf_counter()

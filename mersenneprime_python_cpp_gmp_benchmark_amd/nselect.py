@@ -113,11 +113,17 @@ def candidate_lengths() -> list[int]:
     return [n for n in llhip.supported_lengths(llhip.FP64) if n >= MIN_LARGE_N]
 
 
-def choose_length(p: int, model: dict | None = None, safe_err: float = SAFE_ERR, costs: dict | None = None) -> int:
+MIN_GAIN = 0.03     # leave the conservative length only for a measured gain of 3 % or more
+
+
+def choose_length(p: int, model: dict | None = None, safe_err: float = SAFE_ERR, costs: dict | None = None,
+                  min_gain: float = MIN_GAIN) -> int:
     """The cheapest supported FP64 length whose predicted round-off is below safe_err (cost
     per squaring from the autotuner's table, shortest length when there is no table), never
     beyond the bits-per-digit range the model was trained on; the conservative length if
-    nothing qualifies.  The conservative length is always a candidate when it is safe."""
+    nothing qualifies.  The conservative length is always a candidate when it is safe, and it
+    is kept unless the cheapest candidate is at least min_gain cheaper (the table's costs of
+    neighbouring lengths can differ by less than their run-to-run noise)."""
     model = model or load_model()
     cons = conservative_length(p)
     cands = sorted(set(n for n in candidate_lengths() if 14.0 <= p / n <= 21.0) | {cons})
@@ -132,5 +138,8 @@ def choose_length(p: int, model: dict | None = None, safe_err: float = SAFE_ERR,
     if not safe:
         return cons
     if costs and all(n in costs for n in safe):
-        return min(safe, key=lambda n: (costs[n], n))
+        best = min(safe, key=lambda n: (costs[n], n))
+        if cons in safe and cons in costs and costs[best] > (1.0 - min_gain) * costs[cons]:
+            return cons
+        return best
     return min(safe)
