@@ -255,28 +255,12 @@ def main() -> int:
                                "res64_matches_gmp": interim.get((P24, k24)) == ntt["res64"]
                                if (P24, k24) in interim else None}
         extra["p16777213_5000_iters"] = ent
-        # learned vs plain length on the held-out exponents of the round-off model (primes drawn
-        # with seed 0 from [2^20, 2^25)) where the model picks another length than the rule
+        # C4: the round-off model's held-out exponents (primes drawn with seed 0 from
+        # [2^20, 2^25)): plain GPU form (one per call, conservative length) against one call at
+        # the learned lengths
         try:
-            from mersenneprime_python_cpp_gmp_benchmark_amd import nselect
-            from mersenneprime_python_cpp_gmp_benchmark_amd.roundoff_data import exponents
-            model = nselect.load_model()
-            k4 = 5000
-            diff = [(p, nselect.conservative_length(p), nselect.choose_length(p, model)) for p in exponents("heldout")]
-            diff = [d for d in diff if d[1] != d[2]]
-            tp = tl = 0.0
-            eq = 0
-            for p, nc, nl in diff:
-                rc = run_single(llhip, p, k4, llhip.FP64, nc)
-                rl = run_single(llhip, p, k4, llhip.FP64, nl)
-                tp += rc["wall"]
-                tl += rl["wall"]
-                eq += rc["res64"] == rl["res64"]
-            extra["c4_heldout_learned_vs_plain"] = {
-                "iters": k4, "heldout": len(exponents("heldout")), "learned_differs": len(diff),
-                "seconds_plain": round(tp, 3), "seconds_learned": round(tl, 3),
-                "speedup_vs_plain": round(tp / tl, 3) if tl else None, "res64_equal": eq,
-                "lengths": [[p, nc, nl] for p, nc, nl in diff]}
+            from mersenneprime_python_cpp_gmp_benchmark_amd import c4_eval
+            extra["c4_heldout_5000_iters"] = c4_eval.compare(5000, 0, True, log=log)["summary"]
         except Exception as e:  # noqa: BLE001 -- the model or its data may be absent
             log(f"held-out comparison skipped: {e}")
         # plain GPU form of the batch: one exponent per launch, a sample across the range

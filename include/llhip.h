@@ -90,9 +90,10 @@ int llh_get_batch_families(void);
 int llh_set_batch_cost(uint64_t n, int transform, double us);
 double llh_get_batch_cost(uint64_t n, int transform);
 
-/* llh_batch with every exponent run at length n_force (0 = the engine's choice) and flags
- * bit 0 = no rollback.  No reference counterpart: the autotuner's entry point. */
-int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, uint64_t n_force, int flags,
+/* llh_batch with exponent i run at length n_force[i] (n_force NULL or an entry 0: the engine's
+ * choice) and flags bit 0 = no rollback.  No reference counterpart: the entry point of the
+ * autotuner and of callers that pick lengths themselves (the learned length of nselect). */
+int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, const uint64_t *n_force, int flags,
                 int32_t *verdict, uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used,
                 int32_t *status);
 

@@ -570,77 +570,99 @@ static int get_large_tables(int transform, u64 N, LargeTables &out) {
     return LLH_OK;
 }
 
+
+
+// Result of the large engine's device normalisation (lg_normalize's NormOut).
 struct NormOutH {
     unsigned long long res64;
     int zero;
     int status;
 };
 
-// One exponent on the large engine at length 2^log_n.  Round-off is checked after every
-// block of kGraphBlock squarings; above 0.4 the run stops with kStRoundoff.
-static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
-    const u64 M = transform == kFP64 ? N / 2 : N;
+// One exponent on the large engine at length N, as a resumable job.  Round-off is checked after
+// every block of kGraphBlock squarings; above 0.4 the job stops with kStRoundoff.  begin() allocates and
+// starts, each launch_block() queues up to kGraphBlock squarings (a captured hipGraph) on the
+// job's stream, wait_block() reads their round-off back, finish() normalises.  Several jobs run
+// at once on the library's streams: one exponent of this size leaves most of the GPU idle.
+struct LargeJob {
+    u64 p = 0, iters = 0, N = 0, done = 0, k = 0;
+    int transform = kFP64;
+    hipStream_t st = nullptr;
     LargeLaunch L;
-    if (!N || !large_launch_for(transform, M, L) || (N + 2) * (double)p >= 9.0e15) {
-        r.status = kStNotRun;
-        return LLH_OK;
-    }
-    LargeTables T;
-    int rc = get_large_tables(transform, N, T);
-    if (rc) return rc;
-    const u64 H = (u64)L.h_len, W = (u64)L.w_len;
-    const int log_n = (N & (N - 1)) ? 0 : ceil_log2(N);
-    const size_t esz = transform == kFP64 ? 16 : 8;
-    const u64 nblk = (u64)large_blocks(L);
-    // errs: kGraphBlock round-off slots, then the hand-off fault word
-    const size_t errs_bytes = (kGraphBlock + 1) * 8;
-    // sync: the fused pass's launch counter, then one flag per column block
-    const size_t sync_bytes = 8 + nblk * 4;
-    void *elem = nullptr, *carry = nullptr, *dig = nullptr, *errs = nullptr, *nout = nullptr, *sync = nullptr;
-    HIPCHK(hipMalloc(&elem, M * esz));
-    HIPCHK(hipMalloc(&carry, nblk * H * 8));
-    HIPCHK(hipMalloc(&dig, N * 8));
-    HIPCHK(hipMalloc(&errs, errs_bytes));
-    HIPCHK(hipMalloc(&nout, sizeof(NormOutH)));
-    HIPCHK(hipMalloc(&sync, sync_bytes));
-    hipStream_t st = g_dev.stream;
-    HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
-    HIPCHK(hipMemsetAsync(carry, 0, nblk * H * 8, st));
-    HIPCHK(hipMemsetAsync(sync, 0, sync_bytes, st));
-    if (transform == kFP64) {
-        const double four = 4.0;
-        HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
-    } else {
-        const u64 four = 4;
-        HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
-    }
     LargeArgs A;
-    A.elem = elem; A.carry = carry; A.err = (unsigned long long *)errs;
-    A.twW = T.twW; A.twWi = T.twWi; A.twH = T.twH; A.twHi = T.twHi;
-    A.tmL = T.tmL; A.tmH = T.tmH; A.tmLi = T.tmLi; A.tmHi = T.tmHi;
-    A.wfL = T.wfL; A.wfH = T.wfH; A.wiL = T.wiL; A.wiH = T.wiH;
-    A.p = (u32)p; A.log_n = (u32)log_n; A.log_s = T.log_s; A.log_sw = T.log_sw; A.err_slot = 0;
-    A.n = N; A.inv_n = 1.0 / (double)N;
-    A.p_mod_n = (u32)(p % N); A.p_div_n = (u32)(p / N);
-    A.ticket = (unsigned long long *)sync;
-    A.flags = (unsigned *)((char *)sync + 8);
-    A.fault = (unsigned *)((char *)errs + kGraphBlock * 8);
-
+    void *elem = nullptr, *carry = nullptr, *dig = nullptr, *errs = nullptr, *nout = nullptr, *sync = nullptr;
+    double *eh = nullptr;             // pinned: round-off slots + fault word of the last block
+    NormOutH *noh = nullptr;          // pinned
+    size_t errs_bytes = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    std::vector<double> eh(kGraphBlock + 1);
-    g_err_trace.clear();
-    g_err_trace.reserve(iters);
-    hipEvent_t ev0, ev1;
-    HIPCHK(hipEventCreate(&ev0));
-    HIPCHK(hipEventCreate(&ev1));
-    HIPCHK(hipEventRecord(ev0, st));
-    r.status = kStOk;
-    r.maxerr = 0;
-    u64 done = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<double> trace;
+    BatchResult r = BatchResult();
     bool fault = false;
-    while (done < iters) {
-        const u64 k = std::min<u64>(kGraphBlock, iters - done);
+
+    // 0 = started; kStNotRun in r.status = no kernels for this length
+    int begin(u64 p_, u64 iters_, int transform_, u64 N_, hipStream_t st_) {
+        p = p_; iters = iters_; transform = transform_; N = N_; st = st_;
+        done = 0;
+        r = BatchResult();
+        r.status = kStOk;
+        const u64 M = transform == kFP64 ? N / 2 : N;
+        if (!N || !large_launch_for(transform, M, L) || (N + 2) * (double)p >= 9.0e15) {
+            r.status = kStNotRun;
+            return LLH_OK;
+        }
+        LargeTables T;
+        int rc = get_large_tables(transform, N, T);
+        if (rc) return rc;
+        const u64 H = (u64)L.h_len;
+        const int log_n = (N & (N - 1)) ? 0 : ceil_log2(N);
+        const size_t esz = transform == kFP64 ? 16 : 8;
+        const u64 nblk = (u64)large_blocks(L);
+        // errs: kGraphBlock round-off slots, then the hand-off fault word
+        errs_bytes = (kGraphBlock + 1) * 8;
+        // sync: the fused pass's launch counter, then one flag per column block
+        const size_t sync_bytes = 8 + nblk * 4;
+        HIPCHK(hipMalloc(&elem, M * esz));
+        HIPCHK(hipMalloc(&carry, nblk * H * 8));
+        HIPCHK(hipMalloc(&dig, N * 8));
+        HIPCHK(hipMalloc(&errs, errs_bytes));
+        HIPCHK(hipMalloc(&nout, sizeof(NormOutH)));
+        HIPCHK(hipMalloc(&sync, sync_bytes));
+        HIPCHK(hipHostMalloc((void **)&eh, errs_bytes, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&noh, sizeof(NormOutH), hipHostMallocDefault));
+        HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
+        HIPCHK(hipMemsetAsync(carry, 0, nblk * H * 8, st));
+        HIPCHK(hipMemsetAsync(sync, 0, sync_bytes, st));
+        if (transform == kFP64) {
+            const double four = 4.0;
+            HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+        } else {
+            const u64 four = 4;
+            HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+        }
+        A.elem = elem; A.carry = carry; A.err = (unsigned long long *)errs;
+        A.twW = T.twW; A.twWi = T.twWi; A.twH = T.twH; A.twHi = T.twHi;
+        A.tmL = T.tmL; A.tmH = T.tmH; A.tmLi = T.tmLi; A.tmHi = T.tmHi;
+        A.wfL = T.wfL; A.wfH = T.wfH; A.wiL = T.wiL; A.wiH = T.wiH;
+        A.p = (u32)p; A.log_n = (u32)log_n; A.log_s = T.log_s; A.log_sw = T.log_sw; A.err_slot = 0;
+        A.n = N; A.inv_n = 1.0 / (double)N;
+        A.p_mod_n = (u32)(p % N); A.p_div_n = (u32)(p / N);
+        A.ticket = (unsigned long long *)sync;
+        A.flags = (unsigned *)((char *)sync + 8);
+        A.fault = (unsigned *)((char *)errs + kGraphBlock * 8);
+        trace.clear();
+        trace.reserve(iters);
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        HIPCHK(hipEventRecord(ev0, st));
+        return LLH_OK;
+    }
+
+    bool running() const { return r.status == kStOk && done < iters && !fault; }
+
+    int launch_block() {
+        k = std::min<u64>(kGraphBlock, iters - done);
         HIPCHK(hipMemsetAsync(errs, 0, errs_bytes, st));
         if (k == (u64)kGraphBlock) {
             if (!gexec) {
@@ -659,63 +681,78 @@ static int run_large(u64 p, u64 iters, int transform, u64 N, BatchResult &r) {
         } else {
             HIPCHK(large_squarings(L, A, (int)k, st));
         }
-        HIPCHK(hipMemcpyAsync(eh.data(), errs, errs_bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(eh, errs, errs_bytes, hipMemcpyDeviceToHost, st));
+        return LLH_OK;
+    }
+
+    // after the queued block: round-off bookkeeping and the rollback / fault decisions
+    int wait_block() {
         HIPCHK(hipStreamSynchronize(st));
         unsigned fw;
         std::memcpy(&fw, &eh[kGraphBlock], sizeof fw);
         if (fw) {
             fault = true;
             r.status = kStNotRun;
-            break;
+            return LLH_OK;
         }
         double bmax = 0;
         for (u64 i = 0; i < k; ++i) {
-            g_err_trace.push_back(eh[i]);
+            trace.push_back(eh[i]);
             bmax = std::max(bmax, eh[i]);
         }
         r.maxerr = std::max(r.maxerr, bmax);
         done += k;
-        if (transform == kFP64 && bmax > 0.4) {
-            r.status = kStRoundoff;
-            break;
+        if (transform == kFP64 && bmax > 0.4) r.status = kStRoundoff;
+        return LLH_OK;
+    }
+
+    int finish() {
+        if (!ev0) {                       // never started (no kernels for this length)
+            release();
+            return LLH_OK;
         }
+        noh->res64 = 0; noh->zero = 0; noh->status = 0;
+        if (r.status == kStOk) {
+            HIPCHK(large_apply_carry(L, A, st));
+            HIPCHK(large_normalize(transform, elem, (i64 *)dig, A, nout, st));
+            HIPCHK(hipMemcpyAsync(noh, nout, sizeof(NormOutH), hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipEventRecord(ev1, st));
+        HIPCHK(hipStreamSynchronize(st));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        r.iters_done = done;
+        r.t0 = 0;
+        r.t1 = (u64)((double)ms * 1e-3 * g_dev.wclk_hz);
+        if (r.status == kStOk) {
+            if (noh->status) r.status = kStNotRun;
+            r.res64 = noh->res64;
+            r.verdict = noh->zero;
+        }
+        release();
+        if (fault) {
+            set_error("large engine: a carry hand-off between workgroups timed out");
+            return LLH_ERR_HIP;
+        }
+        return LLH_OK;
     }
-    NormOutH no{0, 0, 0};
-    if (r.status == kStOk) {
-        HIPCHK(large_apply_carry(L, A, st));
-        HIPCHK(large_normalize(transform, elem, (i64 *)dig, A, nout, st));
-        HIPCHK(hipMemcpyAsync(&no, nout, sizeof(no), hipMemcpyDeviceToHost, st));
+
+    void release() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        void *dev[] = {elem, carry, dig, errs, nout, sync};
+        for (void *d : dev)
+            if (d) (void)hipFree(d);
+        if (eh) (void)hipHostFree(eh);
+        if (noh) (void)hipHostFree(noh);
+        gexec = nullptr; graph = nullptr; ev0 = ev1 = nullptr;
+        elem = carry = dig = errs = nout = sync = nullptr;
+        eh = nullptr; noh = nullptr;
     }
-    HIPCHK(hipEventRecord(ev1, st));
-    HIPCHK(hipStreamSynchronize(st));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-    g_last_kernel_ms += ms;
-    g_last_launches += 1;
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    r.iters_done = done;
-    r.t0 = 0;
-    r.t1 = (u64)((double)ms * 1e-3 * g_dev.wclk_hz);
-    if (r.status == kStOk) {
-        if (no.status) r.status = kStNotRun;
-        r.res64 = no.res64;
-        r.verdict = no.zero;
-    }
-    (void)hipFree(elem);
-    (void)hipFree(carry);
-    (void)hipFree(dig);
-    (void)hipFree(errs);
-    (void)hipFree(nout);
-    (void)hipFree(sync);
-    if (fault) {
-        set_error("large engine: a carry hand-off between workgroups timed out");
-        return LLH_ERR_HIP;
-    }
-    return LLH_OK;
-}
+    ~LargeJob() { release(); }
+};
 
 // Lengths the large engine runs, ascending.  FP64: {1, 3, 5, 7, 15} * 2^k for N in
 // [2^15, 2^22]; GL: powers of two 2^15 .. 2^21.
@@ -753,9 +790,113 @@ static u64 next_large_length(u64 N, int transform) {
     return 0;
 }
 
+static const int kLargeJobs = 4;
+
+static int run_large_list(std::vector<Pending> &large, int transform, bool no_rollback, int32_t *verdict,
+                          uint64_t *res64, int32_t *status, uint32_t *n_used, std::vector<double> &secs,
+                          std::vector<double> &errs) {
+    std::sort(large.begin(), large.end(), [](const Pending &a, const Pending &b) {
+        return (double)a.iters * (double)a.n > (double)b.iters * (double)b.n;
+    });
+    size_t next = 0;
+    struct Slot {
+        LargeJob job;
+        size_t item = 0;
+        bool busy = false;
+    };
+    std::vector<Slot> slots(std::min<size_t>(kLargeJobs, large.size()));
+    int rc;
+    // the last exponent to finish leaves its round-off trace (a single exponent: its own)
+    auto record = [&](Slot &s) {
+        const Pending &w = large[s.item];
+        const BatchResult &r = s.job.r;
+        secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
+        errs[w.idx] = std::max(errs[w.idx], r.maxerr);
+        verdict[w.idx] = r.verdict;
+        res64[w.idx] = r.res64;
+        status[w.idx] = r.status;
+        if (n_used) n_used[w.idx] = (uint32_t)s.job.N;
+        g_err_trace = s.job.trace;
+    };
+    // start job for item i in slot s at length N (queues its first block)
+    auto start = [&](Slot &s, size_t i, u64 N) -> int {
+        s.item = i;
+        s.busy = true;
+        int r2 = s.job.begin(large[i].p, large[i].iters, transform, N, g_dev.fstream[&s - slots.data()]);
+        if (r2) return r2;
+        if (s.job.running()) return s.job.launch_block();
+        return LLH_OK;
+    };
+    auto fill = [&](Slot &s) -> int {
+        while (next < large.size()) {
+            const size_t i = next++;
+            if (large[i].n == 0) {
+                status[large[i].idx] = kStNotRun;
+                continue;
+            }
+            return start(s, i, large[i].n);
+        }
+        s.busy = false;
+        return LLH_OK;
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, g_dev.stream));
+    for (Slot &s : slots) {
+        HIPCHK(hipStreamWaitEvent(g_dev.fstream[&s - slots.data()], e0, 0));
+        if ((rc = fill(s))) return rc;
+    }
+    for (bool any = true; any;) {
+        any = false;
+        for (Slot &s : slots) {
+            if (!s.busy) continue;
+            any = true;
+            LargeJob &j = s.job;
+            if (j.running()) {
+                if ((rc = j.wait_block())) return rc;
+                if (j.running()) {
+                    if ((rc = j.launch_block())) return rc;
+                    continue;
+                }
+            }
+            // the job has stopped: finished, rolled back or not run
+            if ((rc = j.finish())) return rc;
+            if (j.r.status == kStRoundoff && transform == kFP64 && !no_rollback) {
+                const u64 nx = next_large_length(j.N, transform);   // rollback: redo longer
+                if (nx) {
+                    const Pending &w = large[s.item];
+                    secs[w.idx] += (double)(j.r.t1 - j.r.t0) / g_dev.wclk_hz;
+                    errs[w.idx] = std::max(errs[w.idx], j.r.maxerr);
+                    if ((rc = start(s, s.item, nx))) return rc;
+                    continue;
+                }
+            }
+            record(s);
+            if ((rc = fill(s))) return rc;
+        }
+    }
+    // device time of the whole list: from e0 to the last stream's end
+    for (size_t i = 1; i < slots.size(); ++i) {
+        HIPCHK(hipEventRecord(e1, g_dev.fstream[i]));
+        HIPCHK(hipStreamWaitEvent(g_dev.stream, e1, 0));
+    }
+    HIPCHK(hipEventRecord(e1, g_dev.stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!slots.empty()) {
+        g_last_kernel_ms += ms;
+        g_last_launches += 1;
+    }
+    return LLH_OK;
+}
+
 static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int32_t *verdict,
                       uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status,
-                      u64 force_n = 0, bool no_rollback = false) {
+                      u64 force_n = 0, bool no_rollback = false, const u64 *force_each = nullptr) {
     if (transform != kFP64 && transform != kNTT) {
         set_error("transform must be LLH_FP64 or LLH_NTT");
         return LLH_ERR_ARG;
@@ -786,12 +927,17 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
             work.push_back(Pending{p, it, i, 0, 1});
             continue;
         }
-        if (force_n) {
-            const int c = batch_class_of(transform, force_n);
-            if (c && batch_fits(p, force_n))
-                work.push_back(Pending{p, it, i, c, force_n});
+        const u64 fn = force_each && force_each[i] ? force_each[i] : force_n;
+        if (fn) {
+            if (fn < 16 || fn > ((u64)1 << 24)) {
+                set_error("forced length must be in [16, 2^24]");
+                return LLH_ERR_ARG;
+            }
+            const int c = batch_class_of(transform, fn);
+            if (c && batch_fits(p, fn))
+                work.push_back(Pending{p, it, i, c, fn});
             else
-                large.push_back(Pending{p, it, i, -1, force_n});
+                large.push_back(Pending{p, it, i, -1, fn});
             continue;
         }
         const u64 bn = batch_length(p, transform);
@@ -801,27 +947,9 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
         }
         work.push_back(Pending{p, it, i, batch_class_of(transform, bn), bn});
     }
-    // exponents beyond the batched engine: one at a time on the large engine (longest first)
-    for (const Pending &w : large) {
-        BatchResult r = BatchResult();
-        if (w.n == 0) { status[w.idx] = kStNotRun; continue; }
-        u64 N = w.n;
-        for (;;) {
-            rc = run_large(w.p, w.iters, transform, N, r);
-            if (rc) return rc;
-            secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
-            errs[w.idx] = std::max(errs[w.idx], r.maxerr);
-            if (r.status == kStRoundoff && transform == kFP64 && !no_rollback) {
-                const u64 nx = next_large_length(N, transform);   // rollback: redo longer
-                if (nx) { N = nx; continue; }
-            }
-            break;
-        }
-        verdict[w.idx] = r.verdict;
-        res64[w.idx] = r.res64;
-        status[w.idx] = r.status;
-        if (n_used) n_used[w.idx] = (uint32_t)N;
-    }
+    // exponents beyond the batched engine: the large engine, up to kLargeJobs at once (one per
+    // stream), longest first; a job that rolls back restarts at the next length in its slot
+    if ((rc = run_large_list(large, transform, no_rollback, verdict, res64, status, n_used, secs, errs))) return rc;
     while (!work.empty()) {
         std::vector<BatchResult> res;
         rc = run_launch(transform, work, res);
@@ -960,16 +1088,12 @@ int llh_set_batch_cost(uint64_t n, int transform, double us) {
 
 double llh_get_batch_cost(uint64_t n, int transform) { return batch_cost(transform, n); }
 
-int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, uint64_t n_force, int flags,
+int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, const uint64_t *n_force, int flags,
                 int32_t *verdict, uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used,
                 int32_t *status) {
-    if (n_force && (n_force < 16 || n_force > ((uint64_t)1 << 24))) {
-        set_error("n_force must be in [16, 2^24]");
-        return LLH_ERR_ARG;
-    }
     try {
-        return batch_impl(ps, n, max_iters, transform, verdict, res64, seconds, maxerr, n_used, status, n_force,
-                          (flags & 1) != 0);
+        return batch_impl(ps, n, max_iters, transform, verdict, res64, seconds, maxerr, n_used, status, 0,
+                          (flags & 1) != 0, n_force);
     } catch (const std::exception &e) {
         set_error(e.what());
         return LLH_ERR_HIP;

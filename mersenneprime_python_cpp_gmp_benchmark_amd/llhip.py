@@ -131,17 +131,20 @@ def batch(ps, iters: int = 0, transform: int = NTT) -> dict:
     return out
 
 
-def batch_n(ps, iters: int = 0, transform: int = FP64, n: int = 0, no_rollback: bool = False) -> dict:
-    """llh_batch_n: every exponent at transform length n (0 = the engine's choice)."""
+def batch_n(ps, iters: int = 0, transform: int = FP64, n=0, no_rollback: bool = False) -> dict:
+    """llh_batch_n: exponent i at transform length n[i] (n a sequence, or one length for all;
+    0 = the engine's choice)."""
     ps = np.ascontiguousarray(np.asarray(ps, dtype=np.uint64))
     k = len(ps)
+    ns = np.ascontiguousarray(np.broadcast_to(np.asarray(n, dtype=np.uint64), (k,)).copy())
     out = {
         "verdict": np.zeros(k, np.int32), "res64": np.zeros(k, np.uint64), "seconds": np.zeros(k, np.float64),
         "maxerr": np.zeros(k, np.float64), "n_used": np.zeros(k, np.uint32), "status": np.zeros(k, np.int32),
     }
     P = ctypes.POINTER
     c = lambda a, t: a.ctypes.data_as(P(t))  # noqa: E731
-    _check(lib().llh_batch_n(c(ps, ctypes.c_uint64), k, iters, transform, n, 1 if no_rollback else 0,
+    _check(lib().llh_batch_n(c(ps, ctypes.c_uint64), k, iters, transform, c(ns, ctypes.c_uint64),
+                             1 if no_rollback else 0,
                              c(out["verdict"], ctypes.c_int32), c(out["res64"], ctypes.c_uint64),
                              c(out["seconds"], ctypes.c_double), c(out["maxerr"], ctypes.c_double),
                              c(out["n_used"], ctypes.c_uint32), c(out["status"], ctypes.c_int32)))

@@ -1,6 +1,7 @@
-"""Parity of the large-exponent engine (residue in HBM, three launches per squaring) with
+"""Parity of the large-exponent engine (residue in HBM, two launches per squaring) with
 the GMP oracle, through the C ABI: interim Res64 at every supported length, both transforms,
-full tests of known Mersenne primes, and the committed interim fixtures."""
+full tests of known Mersenne primes, the committed interim fixtures, and several large
+exponents in flight at once."""
 import pytest
 
 from conftest import load_interim_fixture
@@ -100,3 +101,29 @@ def test_conservative_length_rule_matches_reference():
     L = llhip.lib()
     for p in (1048573, 16777213, 3021377, 20000003, 33554393):
         assert L.llh_fft_length(p, 18.0) == O.fft_length(p, 18.0)
+
+
+def test_heldout_exponents_in_one_call_match_gmp():
+    """The 8 held-out exponents of the round-off model (2^20 .. 2^25) in one batch call run on the
+    large engine several at a time; every Res64 after 20,000 squarings equals GMP's."""
+    rows = load_interim_fixture("c4_gmp_heldout.csv")
+    ps = [r[0] for r in rows]
+    out = llhip.batch(ps, 20000, llhip.FP64)
+    assert list(out["status"]) == [llhip.ST_OK] * len(ps)
+    for i, (p, k, want) in enumerate(rows):
+        assert k == 20000
+        assert int(out["res64"][i]) == want, p
+
+
+@pytest.mark.parametrize("tr", [llhip.FP64, llhip.NTT], ids=["fp64", "ntt"])
+def test_concurrent_large_jobs_equal_single_runs(tr):
+    """Ten large exponents at mixed lengths in one call (more jobs than streams, so slots are
+    refilled) give the Res64 of the same exponents run one per call."""
+    ps = [1048573, 1299709, 2000003, 2750159, 4194301, 5000011, 6291469, 8388593, 1572869, 3145739]
+    k = 400
+    out = llhip.batch(ps, k, tr)
+    for i, p in enumerate(ps):
+        one = llhip.lucas_lehmer_n(p, k, tr, 0)
+        assert one["status"] == llhip.ST_OK and int(out["status"][i]) == llhip.ST_OK, p
+        assert int(out["res64"][i]) == one["res64"], p
+        assert int(out["n_used"][i]) == one["n_used"], p
