@@ -29,17 +29,26 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 COSTS_PATH = os.path.join(HERE, "length_costs.json")
 
 
-def measure(iters: int = 600, min_n: int = 1 << 16) -> dict:
-    out = {}
+LOADED_JOBS = 4     # exponents in flight for the loaded cost (the large engine's streams)
+
+
+def measure(iters: int = 600, min_n: int = 1 << 16) -> tuple[dict, dict]:
+    """(alone, loaded): device microseconds per squaring at every large-engine length, with the
+    exponent alone on the GPU and with LOADED_JOBS exponents of that length in one call."""
+    alone, loaded = {}, {}
     for n in llhip.supported_lengths(llhip.FP64):
         if n < min_n:
             continue
         p = int(17 * n) | 1
         llhip.lucas_lehmer_n(p, 50, llhip.FP64, n, no_rollback=True)       # tables + warm-up
         r = llhip.lucas_lehmer_n(p, iters, llhip.FP64, n, no_rollback=True)
-        out[str(n)] = r["seconds"] / iters * 1e6
-        print(f"N={n:8d} {out[str(n)]:8.2f} us/squaring", flush=True)
-    return out
+        alone[str(n)] = r["seconds"] / iters * 1e6
+        llhip.batch_n([p] * LOADED_JOBS, iters, llhip.FP64, n, no_rollback=True)
+        ms, _ = llhip.last_launch_stats()
+        loaded[str(n)] = ms * 1e3 / (LOADED_JOBS * iters)
+        print(f"N={n:8d} {alone[str(n)]:8.2f} us/squaring alone, {loaded[str(n)]:8.2f} with {LOADED_JOBS} in flight",
+              flush=True)
+    return alone, loaded
 
 
 BATCH_JSON = os.path.join(HERE, "batch_costs.json")
@@ -107,9 +116,13 @@ def write_batch(costs: dict, iters: int) -> None:
             f.write(f"    {{{n}, {tr}, {v:.6g}}},\n")
 
 
-def load_costs(path: str = COSTS_PATH) -> dict:
+def load_costs(path: str = COSTS_PATH, loaded: bool = False) -> dict:
+    """{N: us per squaring}; loaded: the cost with several exponents in flight (one call over
+    many exponents) instead of one exponent alone."""
     with open(path) as f:
-        return {int(k): float(v) for k, v in json.load(f)["us_per_iter"].items()}
+        d = json.load(f)
+    key = "us_per_iter_loaded" if loaded and "us_per_iter_loaded" in d else "us_per_iter"
+    return {int(k): float(v) for k, v in d[key].items()}
 
 
 def main() -> int:
@@ -122,11 +135,11 @@ def main() -> int:
         it = a.iters if a.iters != 600 else 300
         write_batch(measure_batch(it), it)
         return 0
-    costs = measure(a.iters)
+    alone, loaded = measure(a.iters)
     dev = llhip.device_info()
     with open(a.out, "w") as f:
         json.dump({"device": dev["name"] or dev["arch"], "arch": dev["arch"], "iters": a.iters,
-                   "us_per_iter": costs}, f, indent=1)
+                   "us_per_iter": alone, "loaded_jobs": LOADED_JOBS, "us_per_iter_loaded": loaded}, f, indent=1)
     return 0
 
 

@@ -8,7 +8,8 @@ The held-out exponents of tests/golden/c4_exponents.csv (primes drawn with seed 
                 GPU form);
   conservative: all exponents in one call at the conservative lengths (the large engine keeps
                 several exponents in flight);
-  learned:      all exponents in one call at the lengths nselect.choose_length picks.
+  learned:      all exponents in one call at the lengths nselect.choose_length picks with the
+                costs measured with several exponents in flight.
 
 Res64 of every run is compared with the others and with the committed GMP fixture where there
 is one; the device time of each is reported.
@@ -47,7 +48,7 @@ def compare(iters: int = 20000, limit: int = 0, plain: bool = True, log=print) -
     gmp = gmp_fixture()
     model = nselect.load_model()
     nc = [nselect.conservative_length(p) for p in ps]
-    nl = [nselect.choose_length(p, model) for p in ps]
+    nl = [nselect.choose_length(p, model, loaded=True) for p in ps]
     t_plain = None
     single = None
     if plain:

@@ -117,20 +117,21 @@ MIN_GAIN = 0.03     # leave the conservative length only for a measured gain of 
 
 
 def choose_length(p: int, model: dict | None = None, safe_err: float = SAFE_ERR, costs: dict | None = None,
-                  min_gain: float = MIN_GAIN) -> int:
+                  min_gain: float = MIN_GAIN, loaded: bool = False) -> int:
     """The cheapest supported FP64 length whose predicted round-off is below safe_err (cost
     per squaring from the autotuner's table, shortest length when there is no table), never
     beyond the bits-per-digit range the model was trained on; the conservative length if
     nothing qualifies.  The conservative length is always a candidate when it is safe, and it
     is kept unless the cheapest candidate is at least min_gain cheaper (the table's costs of
-    neighbouring lengths can differ by less than their run-to-run noise)."""
+    neighbouring lengths can differ by less than their run-to-run noise).  loaded: price the
+    lengths with several exponents in flight (one call over many exponents)."""
     model = model or load_model()
     cons = conservative_length(p)
     cands = sorted(set(n for n in candidate_lengths() if 14.0 <= p / n <= 21.0) | {cons})
     if costs is None:
         try:
             from .autotune import load_costs
-            costs = load_costs()
+            costs = load_costs(loaded=loaded)
         except (OSError, ValueError, KeyError):
             costs = {}
     err = predict_err(np.full(len(cands), p), np.array(cands), model)
