@@ -144,6 +144,18 @@ __device__ int normalize_res64(i64 *d, u32 p, u32 N, u64 &res, int &zero) {
     return 0;
 }
 
+LL_HD constexpr int ilog2_ceil(u32 v) { return v <= 1 ? 0 : 1 + ilog2_ceil((v + 1) / 2); }
+
+// A wave-uniform pointer, moved into SGPRs (readfirstlane) so loads through it take a scalar
+// base and a 32-bit vector offset.  run_class is a real call and receives the class descriptor
+// through memory, which would otherwise leave its table pointers in VGPRs.
+template <class T>
+LL_DEV gptr<T> uniform_table(const void *p) {
+    const u64 v = (u64)p;
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)v), hi = __builtin_amdgcn_readfirstlane((u32)(v >> 32));
+    return (gptr<T>)(((u64)hi << 32) | lo);
+}
+
 // Bits and weight exponents of the two digits 2j, 2j + 1 of FP64 element j.
 struct Pair {
     int b0, b1;
@@ -174,18 +186,21 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 
     const int tid = threadIdx.x, g = tid / G, t = tid % G;
     T *lds = reinterpret_cast<T *>(g_smem<WG>) + g * REG;
-    const int log_n = cd.log_n;                   // powers of two only
-    const u32 nmask = N - 1;
-    const gptr<W> tw = as_global<W>(cd.tw);
-    const gptr<W> twp = as_global<W>(cd.twp);
-    const gptr<W> twpi = IS_GL ? as_global<W>(cd.twpi) : twp;
-    const gptr<WT> wf = as_global<WT>(cd.wf);
-    const gptr<WT> wi = as_global<WT>(cd.wi);
-    const u32 smask = (1u << cd.log_s) - 1;
-    const int log_s = cd.log_s;
+    // digit count and weight-table split are fixed by the class (the host builds the two-level
+    // weights with the same S = 2^ceil(ceil(log2 N) / 2))
+    constexpr int log_n = ilog2_ceil(N);          // exact for the powers of two, the only users
+    constexpr u32 nmask = N - 1;
+    constexpr int log_s = (ilog2_ceil(N) + 1) / 2;
+    constexpr u32 smask = (1u << log_s) - 1;
+    const gptr<W> tw = uniform_table<W>(cd.tw);
+    const gptr<W> twp = uniform_table<W>(cd.twp);
+    const gptr<W> twpi = IS_GL ? uniform_table<W>(cd.twpi) : twp;
+    const gptr<WT> wf = uniform_table<WT>(cd.wf);
+    const gptr<WT> wi = uniform_table<WT>(cd.wi);
     if constexpr (!IS_GL) {
+        const gptr<double> wt = uniform_table<double>(cd.wtab);
 #pragma unroll
-        for (int i = tid; i < 4 * llh::kWtabSlot; i += WG) g_wtab[i] = ld_table(as_global<double>(cd.wtab), (u32)i);
+        for (int i = tid; i < 4 * llh::kWtabSlot; i += WG) g_wtab[i] = ld_table(wt, (u32)i);
     }
     // weight of exponent e (forward: slots 0, 1; inverse: slots 2, 3)
     auto wlds = [&](u32 e, int inv) {

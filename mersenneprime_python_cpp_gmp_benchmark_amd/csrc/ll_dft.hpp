@@ -241,7 +241,7 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
         }
     }
     if constexpr (L > 1) {
-        const int k = t % L;
+        const int k = (int)((u32)t % (u32)L);     // t >= 0: unsigned remainder
         constexpr int TSTEP = M / (r * L);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
@@ -266,8 +266,8 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
                 for (int q = 0; q < r; ++q) lds[wb + G * b * (r + 1) + q] = x[b * r + q];
         } else if constexpr (L % P == 0) {
             // (t - k) r, G b r and q L are multiples of P; k < L adds k + k / P
-            const int k = t % L;
-            const int wb = (t - k) * r / P * (P + 1) + k + (k >> PS);
+            const int k = (int)((u32)t % (u32)L);
+            const int wb = (int)((u32)(t - k) * (u32)r / (u32)P * (u32)(P + 1)) + k + (k >> PS);
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -327,7 +327,7 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
     constexpr bool FAST = G % L == 0 && G % P == 0 && NBF % P == 0 && (L == 1 ? r == P : L % P == 0);
     T y[NB][r];
     t = ll_opaque(t);
-    const int kt = t % L;
+    const int kt = (int)((u32)t % (u32)L);
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -354,7 +354,7 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
     }
     __syncthreads();
     // output q of butterfly i goes to (i - k) r + k + q L
-    const int wb = L == 1 ? t * (r + 1) : (t - kt) * r / P * (P + 1) + kt + (kt >> PS);
+    const int wb = L == 1 ? t * (r + 1) : (int)((u32)(t - kt) * (u32)r / (u32)P * (u32)(P + 1)) + kt + (kt >> PS);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int i = t + G * b;
