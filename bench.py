@@ -2,21 +2,25 @@
 """Headline benchmark: Lucas-Lehmer iterations/s of the MI355X engines.
 
 Default workload (one "step"): the full Lucas-Lehmer test of every prime 2 <= p <= 100001
-(9592 exponents, sum(p-2) = 454,377,353 squarings) in ONE persistent launch of the batched
-FP64 IBDWT engine, every Res64 checked against the committed GMP fixture.
+(9592 exponents, sum(p-2) = 454,377,353 squarings) in one call of the batched FP64 IBDWT
+engine (one persistent kernel per length family, run concurrently), every Res64 checked
+against the committed GMP fixture.
 
 Prints one JSON line: value = iterations/s of the whole step (the only input is the
 exponent list; residues are created and stay on the GPU), plus
-  roofline      the batch kernel against MI355X's FP64 vector peak (78.6 TFLOP/s):
+  roofline      the batch kernels against MI355X's FP64 vector peak (78.6 TFLOP/s):
                 algorithmic flops per launch = sum over exponents of (p-2)(5 N log2 N + 6.5 N)
-                divided by the kernel time (HIP events on the library's stream); traffic from
-                the committed rocprofv3 PMC summary when present.
+                at the length each ran, divided by the device time of the call (HIP events
+                around the fork-join of the family launches); traffic from the committed
+                rocprofv3 PMC summary when present.
   cpu_baseline  the CPU oracle's GMP engine (restates lucaslehmer.cpp:29-56), one exponent per
                 thread (PerfNumMultiCLL.py:189-193), on a bounded sample of the same list.
   extra         the other configurations, each measured briefly: the single large exponent at
                 p = 1,048,573 (N = 2^16), p = 16,777,213 on the FP64 engine at the conservative
-                length and at the learned length, the same on the exact NTT, the batched NTT
-                over the same 9592 exponents, and the plain one-exponent-per-launch form.
+                length and at the learned length, the same on the exact NTT, the round-off
+                model's 64 held-out exponents (plain one-per-call conservative form against one
+                call at the learned lengths, 5000 squarings each), and the plain
+                one-exponent-per-launch form of the batch.
 
 --workload selects one configuration alone (used for the rocprofv3 runs).  The library
 calls are synchronous (they wait on the library's own HIP stream), so every timed region

@@ -1,26 +1,28 @@
 // ll_large.hip -- Lucas-Lehmer for one large exponent: the residue lives in HBM and each
-// squaring is three kernel launches of a four-step transform.
+// squaring is two kernel launches of a four-step transform.
 //
 // Counterpart of the FP64 loop lucaslehmer.cpp:281-311 (and of the NTT loop of
 // cuda/ll_ntt.cu:91-163, which the reference only runs up to N = 8192) for transform lengths
-// 2^15 .. 2^22 elements.  Layout: the M transform elements form an H x W matrix, element
+// 2^15 .. 2^21 elements.  Layout: the M transform elements form an H x W matrix, element
 // j = w + W h (row h, column w; rows contiguous).  With k = k_h + H k_w:
 //
 //   Z[k] = sum_w w_W^{w k_w} w_M^{w k_h} sum_h z[w + W h] w_H^{h k_h}
 //
-//  K1 cols_fwd  (blocks of CW columns): carry-in of the previous squaring, weights,
-//               H-point FFT per column, twiddle w_M^{w k_h}; in place.
-//  K2 rows      (rows k_h and H - k_h together): W-point FFT per row, square (FP64: the
-//               packed real spectrum pairs row k_h with row H - k_h), inverse W-point FFT.
-//  K3 cols_inv  (blocks of CW columns): twiddle w_M^{-w k_h}, inverse H-point FFT, unweight,
-//               round, per-iteration max round-off, -2, and an exact balanced carry along each
-//               row segment of CW elements; the segment's carry-out goes to a carry buffer
-//               and K1 of the next squaring adds it to the next segment (the last segment of
-//               row h feeds row h + 1, and the top one wraps to digit 0: 2^p == 1).
+//  rows       (rows k_h and H - k_h together): W-point FFT per row, square (FP64: the packed
+//             real spectrum pairs row k_h with row H - k_h), inverse W-point FFT.
+//  cols_fused (blocks of kCW columns): twiddle w_M^{-w k_h}, inverse H-point FFT, unweight,
+//             round, per-iteration max round-off, -2, an exact balanced carry along each row
+//             segment of kCW elements; the segment's carry-out is handed to the next block
+//             inside the launch (the last segment of row h feeds row h + 1, and the top one
+//             wraps to digit 0: 2^p == 1); then the carry-in with a re-split, weights, the
+//             forward H-point FFT and the twiddle w_M^{w k_h} of the next squaring.
+//  cols_fwd / cols_inv: the forward and the inverse half alone, which open and close a block
+//             of squarings (the carries between blocks wait in a buffer).
 //
 // Digits: FP64 packs digits 2j, 2j+1 into element j (N = 2M); GL holds digit j in element j
 // (N = M).  Weights and the w_M twiddles come from two-level tables (product of a low and
-// a high table entry), so every table stays cache resident at every length.
+// a high table entry), so every table stays cache resident at every length; the FFT
+// twiddles are staged in LDS in per-pass order.
 #include "ll_common.hpp"
 #include "ll_dft.hpp"
 #include "ll_internal.hpp"
