@@ -572,6 +572,34 @@ static int get_large_tables(int transform, u64 N, LargeTables &out) {
 
 
 
+// Buffers of finished large-engine jobs, kept for the next job of the same size: a hipFree
+// between jobs would wait for the work the other streams still have in flight.
+struct BufCache {
+    std::multimap<size_t, void *> dev, host;
+    void *get(size_t bytes, bool pinned) {
+        auto &m = pinned ? host : dev;
+        auto it = m.find(bytes);
+        if (it == m.end()) return nullptr;
+        void *p = it->second;
+        m.erase(it);
+        return p;
+    }
+    void put(void *p, size_t bytes, bool pinned) {
+        if (p) (pinned ? host : dev).insert({bytes, p});
+    }
+};
+static BufCache g_bufs;
+static int cached_malloc(void **p, size_t bytes) {
+    if ((*p = g_bufs.get(bytes, false))) return LLH_OK;
+    HIPCHK(hipMalloc(p, bytes));
+    return LLH_OK;
+}
+static int cached_host_malloc(void **p, size_t bytes) {
+    if ((*p = g_bufs.get(bytes, true))) return LLH_OK;
+    HIPCHK(hipHostMalloc(p, bytes, hipHostMallocDefault));
+    return LLH_OK;
+}
+
 // Result of the large engine's device normalisation (lg_normalize's NormOut).
 struct NormOutH {
     unsigned long long res64;
@@ -594,6 +622,7 @@ struct LargeJob {
     double *eh = nullptr;             // pinned: round-off slots + fault word of the last block
     NormOutH *noh = nullptr;          // pinned
     size_t errs_bytes = 0;
+    size_t bytes_elem = 0, bytes_carry = 0, bytes_dig = 0, bytes_sync = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -623,14 +652,16 @@ struct LargeJob {
         errs_bytes = (kGraphBlock + 1) * 8;
         // sync: the fused pass's launch counter, then one flag per column block
         const size_t sync_bytes = 8 + nblk * 4;
-        HIPCHK(hipMalloc(&elem, M * esz));
-        HIPCHK(hipMalloc(&carry, nblk * H * 8));
-        HIPCHK(hipMalloc(&dig, N * 8));
-        HIPCHK(hipMalloc(&errs, errs_bytes));
-        HIPCHK(hipMalloc(&nout, sizeof(NormOutH)));
-        HIPCHK(hipMalloc(&sync, sync_bytes));
-        HIPCHK(hipHostMalloc((void **)&eh, errs_bytes, hipHostMallocDefault));
-        HIPCHK(hipHostMalloc((void **)&noh, sizeof(NormOutH), hipHostMallocDefault));
+        bytes_elem = M * esz;
+        bytes_carry = nblk * H * 8;
+        bytes_dig = N * 8;
+        bytes_sync = sync_bytes;
+        if ((rc = cached_malloc(&elem, bytes_elem)) || (rc = cached_malloc(&carry, bytes_carry)) ||
+            (rc = cached_malloc(&dig, bytes_dig)) || (rc = cached_malloc(&errs, errs_bytes)) ||
+            (rc = cached_malloc(&nout, sizeof(NormOutH))) || (rc = cached_malloc(&sync, sync_bytes)) ||
+            (rc = cached_host_malloc((void **)&eh, errs_bytes)) ||
+            (rc = cached_host_malloc((void **)&noh, sizeof(NormOutH))))
+            return rc;
         HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
         HIPCHK(hipMemsetAsync(carry, 0, nblk * H * 8, st));
         HIPCHK(hipMemsetAsync(sync, 0, sync_bytes, st));
@@ -742,11 +773,14 @@ struct LargeJob {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        void *dev[] = {elem, carry, dig, errs, nout, sync};
-        for (void *d : dev)
-            if (d) (void)hipFree(d);
-        if (eh) (void)hipHostFree(eh);
-        if (noh) (void)hipHostFree(noh);
+        g_bufs.put(elem, bytes_elem, false);
+        g_bufs.put(carry, bytes_carry, false);
+        g_bufs.put(dig, bytes_dig, false);
+        g_bufs.put(errs, errs_bytes, false);
+        g_bufs.put(nout, sizeof(NormOutH), false);
+        g_bufs.put(sync, bytes_sync, false);
+        g_bufs.put(eh, errs_bytes, true);
+        g_bufs.put(noh, sizeof(NormOutH), true);
         gexec = nullptr; graph = nullptr; ev0 = ev1 = nullptr;
         elem = carry = dig = errs = nout = sync = nullptr;
         eh = nullptr; noh = nullptr;
