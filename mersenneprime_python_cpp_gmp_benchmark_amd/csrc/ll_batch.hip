@@ -206,42 +206,33 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     auto wlds = [&](u32 e, int inv) {
         return g_wtab[(2 * inv) * llh::kWtabSlot + (e & smask)] * g_wtab[(2 * inv + 1) * llh::kWtabSlot + (e >> log_s)];
     };
-    // Digits 2j, 2j + 1 of element j = t + G s, s = 0 .. R-1, in order: powers of two from
-    // shifts and masks, any N from the residue walk (r = 2 j p mod N, step 2 G p mod N).
+    // Digits 2j, 2j + 1 of element j = t + G s, s = 0 .. R-1, in order.  With P = p mod N and
+    // F = p div N (p > 64 is prime, so coprime to N, and 0 < P < N), the digit of weight
+    // exponent e ((-d p) mod N for digit index d) has F + [e < P] bits, and the next digit's
+    // exponent is (e - P) mod N; element j + G's low digit has (e - 2 G p) mod N.  This equals
+    // the residue walk of DigitWalk digit for digit.
     struct Layout {
-        u32 p;
-        DigitWalk dw;
-        u32 step, r;
+        u32 P, F, step, e;
+    };
+    auto sub_mod = [](u32 a, u32 b) -> u32 {      // (a - b) mod N for a, b < N
+        if constexpr (P2) return (a - b) & nmask;
+        else return a < b ? a - b + N : a - b;
     };
     auto layout = [&](u32 pp, int tt) {
         Layout L;
-        L.p = pp;
-        if constexpr (!P2) {
-            L.dw = DigitWalk{N, pp % N, pp / N};
-            L.step = (2u * (u32)G * pp) % N;
-            L.r = (2u * (u32)tt * pp) % N;
-        }
+        L.P = pp % N;
+        L.F = pp / N;
+        L.step = (2u * (u32)G * L.P) % N;
+        L.e = sub_mod(0u, (2u * (u32)tt * L.P) % N);
         return L;
     };
-    auto pair_at = [&](Layout &L, int tt, int s) {
+    auto pair_at = [&](Layout &L) {
         Pair q;
-        if constexpr (P2) {
-            const u32 j = (u32)tt + (u32)G * (u32)s;
-            const u32 dp0 = 2u * j * L.p;
-            const u32 c0 = (dp0 + N - 1) >> log_n, c1 = (dp0 + L.p + N - 1) >> log_n,
-                      c2 = (dp0 + 2u * L.p + N - 1) >> log_n;
-            q.b0 = (int)(c1 - c0);
-            q.b1 = (int)(c2 - c1);
-            q.e0 = (0u - dp0) & nmask;
-            q.e1 = (0u - dp0 - L.p) & nmask;
-        } else {
-            const u32 r0 = L.r, r1 = L.dw.next(r0);
-            q.b0 = L.dw.bits(r0);
-            q.b1 = L.dw.bits(r1);
-            q.e0 = L.dw.wexp(r0);
-            q.e1 = L.dw.wexp(r1);
-            L.r = L.dw.add(r0, L.step);
-        }
+        q.e0 = L.e;
+        q.e1 = sub_mod(q.e0, L.P);
+        q.b0 = (int)(L.F + (q.e0 < L.P ? 1u : 0u));
+        q.b1 = (int)(L.F + (q.e1 < L.P ? 1u : 0u));
+        L.e = sub_mod(q.e0, L.step);
         return q;
     };
 
@@ -292,7 +283,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 Layout L = layout(ll_opaque(p), tw0);
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
-                    const Pair q = pair_at(L, tw0, s);
+                    const Pair q = pair_at(L);
                     x[s] = cmk(x[s].re * wlds(q.e0, 0), x[s].im * wlds(q.e1, 0));
                 }
             }
@@ -330,7 +321,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const u32 j = tu + G * s;
-                    const Pair q = pair_at(L, tu, s);
+                    const Pair q = pair_at(L);
                     const double v0 = x[s].re * wlds(q.e0, 1), v1 = x[s].im * wlds(q.e1, 1);
                     double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
                     eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
@@ -361,7 +352,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 for (int s = 0; s < R; ++s) {
                     const u32 j = tc + G * s;
                     const double cin = ldsd[j == 0 ? M - 1 : j - 1];
-                    const Pair q = pair_at(L, tc, s);
+                    const Pair q = pair_at(L);
                     double x0 = x[s].re + cin;
                     const double cC = floor(__fma_rn(x0, ldexp(1.0, -q.b0), 0.5));
                     x0 = __fma_rn(-cC, ldexp(1.0, q.b0), x0);
@@ -401,9 +392,8 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
             for (int s = 0; s < R; ++s) {
                 const u32 j = tu + G * s;
-                const u32 dp = j * pu;
-                const u32 e = (0u - dp) & nmask;
-                const int b = (int)(((dp + pu + N - 1) >> log_n) - ((dp + N - 1) >> log_n));
+                const u32 e = (0u - j * pu) & nmask;
+                const int b = (int)((pu >> log_n) + (e < (pu & nmask) ? 1u : 0u));   // as pair_at
                 i64 v = gl_to_i64(gl_mul(x[s], ld_table(wi, e)));
                 if (j == 0 && has) v -= 2;
                 const i64 c = (v + ((i64)1 << (b - 1))) >> b;
@@ -425,8 +415,8 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                     const i64 cin = ldsi[(j - 1) & nmask];
                     i64 v = (i64)x[s] + cin;
                     if (round == 0) {
-                        const u32 dp = j * pc;
-                        const int b = (int)(((dp + pc + N - 1) >> log_n) - ((dp + N - 1) >> log_n));
+                        const u32 e = (0u - j * pc) & nmask;
+                        const int b = (int)((pc >> log_n) + (e < (pc & nmask) ? 1u : 0u));
                         const i64 c = (v + ((i64)1 << (b - 1))) >> b;
                         v -= c << b;
                         cout[s] = c;

@@ -116,6 +116,29 @@ def test_digit_layout_invariants():
         assert all(0 <= e < n for e in we)
 
 
+BATCH_M = (16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 192, 384, 768, 1536, 3072,
+           320, 640, 1280, 2560, 448, 896, 1792, 3584)
+
+
+def test_digit_bits_closed_form():
+    """The batch engine's digit walk: digit d of weight exponent e = (-d p) mod N has
+    p div N + [e < p mod N] bits (p prime and coprime to N), checked against the ceiling
+    definition (lucaslehmer.cpp:262-266) and the oracle's layout at every batch length."""
+    rng = random.Random(11)
+    cands = [q for q in O.primes(65, 180000)]
+    for m in BATCH_M:
+        for n in (2 * m, m):
+            ps = [q for q in cands if n < q < 21 * n]
+            for p in rng.sample(ps, min(6, len(ps))) + [n + 1]:
+                bits, we = O.digit_layout(p, n)
+                P, F = p % n, p // n
+                for d in range(n):
+                    e = (-d * p) % n
+                    want = -(-(d + 1) * p // n) - (-(-d * p // n))
+                    assert F + (e < P) == want == bits[d], (n, p, d)
+                    assert we[d] == e, (n, p, d)
+
+
 def test_size_rules():
     # lucaslehmer.cpp:244-254 at the configurations' exponents
     assert O.fft_length(1048573) == 65536
