@@ -46,7 +46,9 @@ struct Ctl {
     u32 item[kMaxGroups];
     u32 bad[kMaxGroups];
     unsigned long long err[kMaxGroups];
+    u32 ev, any;
 };
+constexpr u32 kCheck = 64;          // squarings between round-off checks
 
 // Size classes.  Powers of two: radices R1 .. R4 (first = last = R; 0 = no pass).  Others:
 // the hybrid schedule 8, H1, [H2], 8 (H1 odd; register radix-8 first and last passes, the
@@ -236,6 +238,13 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
         return q;
     };
 
+    // v = c 2^b + d with the balanced digit |d| <= 2^(b-1): both steps exact for integer v
+    // below 2^53 (the scaling is a power of two, d is the exact difference)
+    auto split = [](double v, int b, double &c) {
+        c = __builtin_rint(ldexp(v, -b));
+        return v - ldexp(c, b);
+    };
+
     u32 item = kNone, p = N + 1;
     u64 target = 0, it = 0, t0 = 0;
     double emax = 0.0;
@@ -262,6 +271,28 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
         t0 = wall_clock64();
     };
 
+    // Squarings to the next workgroup event (a group reaches its target, or the periodic
+    // round-off check every kCheck squarings) and whether any group has an exponent.  Both are
+    // uniform over the workgroup, so the squaring loop itself needs no reduction; a group whose
+    // round-off went above the limit is caught at the next event (its maximum is sticky).
+    u32 ev = 1;
+    bool any = false;
+    auto schedule = [&]() {
+        __syncthreads();
+        if (tid == 0) {
+            ctl.ev = kCheck;
+            ctl.any = 0;
+        }
+        __syncthreads();
+        if (t == 0 && item != kNone) {
+            atomicMin(&ctl.ev, target > it + kCheck ? kCheck : (target > it ? (u32)(target - it) : 1u));
+            ctl.any = 1;
+        }
+        __syncthreads();
+        ev = ctl.ev;
+        any = ctl.any != 0;
+    };
+
     // first fetch for every group
     if (t == 0) {
         u32 k = atomicAdd(counter, 1u);
@@ -270,10 +301,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     __syncthreads();
     item = ctl.item[g];
     init_item();
+    schedule();
 
-    for (;;) {
+    while (any) {
         const bool has = item != kNone;
-        if (!__syncthreads_or(has)) break;
         double eit = 0.0;
 
         if constexpr (!IS_GL) {
@@ -326,11 +357,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                     double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
                     eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
                     if (j == 0 && has) r0 -= 2.0;
-                    const double cA = floor(__fma_rn(r0, ldexp(1.0, -q.b0), 0.5));
-                    const double x0 = __fma_rn(-cA, ldexp(1.0, q.b0), r0);
+                    double cA, cB;
+                    const double x0 = split(r0, q.b0, cA);
                     r1 += cA;
-                    const double cB = floor(__fma_rn(r1, ldexp(1.0, -q.b1), 0.5));
-                    const double x1 = __fma_rn(-cB, ldexp(1.0, q.b1), r1);
+                    const double x1 = split(r1, q.b1, cB);
                     x[s] = cmk(x0, x1);
                     cout[s] = cB;
                 }
@@ -353,12 +383,9 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                     const u32 j = tc + G * s;
                     const double cin = ldsd[j == 0 ? M - 1 : j - 1];
                     const Pair q = pair_at(L);
-                    double x0 = x[s].re + cin;
-                    const double cC = floor(__fma_rn(x0, ldexp(1.0, -q.b0), 0.5));
-                    x0 = __fma_rn(-cC, ldexp(1.0, q.b0), x0);
-                    double x1 = x[s].im + cC;
-                    const double cD = floor(__fma_rn(x1, ldexp(1.0, -q.b1), 0.5));
-                    x1 = __fma_rn(-cD, ldexp(1.0, q.b1), x1);
+                    double cC, cD;
+                    const double x0 = split(x[s].re + cin, q.b0, cC);
+                    const double x1 = split(x[s].im + cC, q.b1, cD);
                     x[s] = cmk(x0, x1);
                     cout[s] = cD;
                 }
@@ -430,10 +457,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             ++it;
             emax = fmax(emax, eit);
         }
-        const bool done = has && it >= target;
-        const bool bad = has && eit > kErrLimit;
-        if (__syncthreads_or(done || bad)) {
-            // ---- finish phase: flags, error reduction, exact normalisation, refill
+        if (--ev == 0) {
+            const bool done = has && it >= target;
+            const bool bad = has && emax > kErrLimit;
+            // ---- event: flags, error reduction, exact normalisation, refill
             if (t == 0) { ctl.bad[g] = 0; ctl.err[g] = 0; }
             __syncthreads();
             if (bad) ctl.bad[g] = 1;
@@ -481,6 +508,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 item = ctl.item[g];
                 init_item();
             }
+            schedule();
         }
     }
 }

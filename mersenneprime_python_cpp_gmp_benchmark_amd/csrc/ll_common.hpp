@@ -67,10 +67,12 @@ LL_DEV gptr<T> as_global(const void *p) {
     return (gptr<T>)p;
 }
 
-// Load element `idx` of a read-only table.
+// Load element `idx` of a read-only table.  The address is the base plus a 32-bit byte
+// offset, which keeps a uniform base in SGPRs (global_load vaddr, saddr) with no 64-bit
+// address arithmetic per lane; tables are far below 4 GB.
 template <class T>
 LL_DEV T ld_table(gptr<T> base, u32 idx) {
-    return base[idx];
+    return *(gptr<T>)((const char LL_GLOBAL *)base + idx * (u32)sizeof(T));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -92,8 +94,8 @@ LL_DEV C64 cmulc(C64 a, C64 b) {
 LL_DEV C64 csqr(C64 a) { return cmk(__fma_rn(a.re, a.re, -a.im * a.im), 2.0 * a.re * a.im); }
 LL_DEV C64 cconj(C64 a) { return cmk(a.re, -a.im); }
 LL_DEV C64 ld_table(gptr<C64> base, u32 idx) {
-    gptr<double> d = (gptr<double>)base;
-    return cmk(d[2 * idx], d[2 * idx + 1]);
+    gptr<double> d = (gptr<double>)((const char LL_GLOBAL *)base + idx * 16u);
+    return cmk(d[0], d[1]);
 }
 
 // ------------------------------------------------------------------------------------------
