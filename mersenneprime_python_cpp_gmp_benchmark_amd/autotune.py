@@ -6,7 +6,7 @@ The learned length choice (nselect.choose_length) uses this table to run the fas
 the round-off model deems safe -- not merely the shortest.
 
 Batched engine (--batch): for each batched length N the GPU is filled with exponents of that
-length (the largest prime within 20 bits per digit, or the exactness bound for the NTT) and
+length (the largest prime within the batch cap of bits per digit, or the exactness bound for the NTT) and
 the cost is the device time per exponent-squaring.  The library's length selector runs the
 cheapest length within the bits-per-digit cap; the table is written as
 csrc/ll_batch_costs.inc (the library's defaults) and batch_costs.json.
@@ -67,9 +67,9 @@ def _primes(lo: int, hi: int) -> list:
 
 
 def batch_exponent(n: int, transform: int) -> int:
-    """Largest prime the batched engine may run at length n (20 bits per digit for FP64)."""
+    """Largest prime the batched engine may run at length n (the batch cap for FP64)."""
     if transform == llhip.FP64:
-        hi = int(20 * n)
+        hi = int(llhip.get_batch_bpw() * n)
     else:
         hi = n * max(1, (63 - (n.bit_length() - 1)) // 2) - n // 2
     hi = min(hi, (1 << 32) // (n + 2) - 1)

@@ -75,10 +75,11 @@ static std::vector<double> g_err_trace;     // per-iteration max round-off of th
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
 static std::atomic<double> g_bpw{18.0};        // reference knob: the conservative length rule
-// Bits-per-digit cap of the batched FP64 engine (lengths up to 8192).  Its worst round-off
-// over every prime <= 100001 is about 0.05 at 20 bits/digit, far below the 0.4 rollback
-// threshold, so short lengths take more bits than the reference's 18.
-static std::atomic<double> g_batch_bpw{20.0};
+// Bits-per-digit cap of the batched FP64 engine (lengths up to 8192).  The worst round-off over
+// every prime <= 100001 roughly doubles per half bit: 0.05 at 19.75-20, 0.10 at 20.25-20.5,
+// 0.21 at 20.75-21, 0.40 at 21.25-21.5 bits/digit (then rollbacks start).  21 keeps it at half
+// the 0.4 rollback threshold, so short lengths take more bits than the reference's 18.
+static std::atomic<double> g_batch_bpw{21.0};
 // Length families the batched engine may pick (bit f: family f; powers of two always).
 static std::atomic<int> g_batch_families{0xF};
 

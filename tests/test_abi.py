@@ -23,9 +23,9 @@ def test_host_only_abi_calls():
     assert L.llh_fft_length(16777213, 18.0) == 1 << 20
     assert L.llh_ntt_length(100001) == 4096                   # cuda/ll_ntt.cu:176-180
     assert L.llh_batch_length(100001, llhip.NTT) == 4096
-    assert L.llh_batch_length(9689, llhip.FP64) == 512        # 18.9 bits per digit <= 20
+    assert L.llh_batch_length(9689, llhip.FP64) == 512        # 18.9 bits per digit <= 21
     assert L.llh_max_exponent(llhip.NTT) == 4096 * 25
-    assert llhip.get_bpw() == 18.0 and llhip.get_batch_bpw() == 20.0
+    assert llhip.get_bpw() == 18.0 and llhip.get_batch_bpw() == 21.0
 
 
 def test_module_surface_and_host_functions():

@@ -93,7 +93,7 @@ def _fp64_ladder():
 
 def test_every_batch_length_matches_gmp():
     """Every FP64 batch length (powers of two and 3, 5, 7 times a power of two), forced onto the
-    three largest primes within 20 bits per digit, in one batch per length: Res64 after 1500
+    three largest primes within the batch cap of bits per digit, in one batch per length: Res64 after 1500
     squarings (and, for the shorter lengths, after the full test) must equal GMP's."""
     from oracle import oracle as O
     lad = _fp64_ladder()
@@ -101,7 +101,7 @@ def test_every_batch_length_matches_gmp():
     for n in lad:
         if n < 64:
             continue
-        ps = [p for p in O.primes(max(65, int(n * 14)), int(n * 20)) if (n + 2) * p < (1 << 32)][-3:]
+        ps = [p for p in O.primes(max(65, int(n * 14)), int(n * llhip.get_batch_bpw())) if (n + 2) * p < (1 << 32)][-3:]
         out = llhip.batch_n(ps, 1500, llhip.FP64, n, no_rollback=True)
         assert (out["status"] == llhip.ST_OK).all(), (n, ps, list(out["status"]))
         assert (out["n_used"] == n).all(), n
