@@ -164,7 +164,9 @@ struct Pair {
     u32 e0, e1;
 };
 
-template <bool IS_GL, int WG, int C>
+// TWO: the FP64 carry takes two hand-off rounds (ClassDesc::carry2, set by the host for the
+// exponents with fewer than log2 N + 3 bits per digit); otherwise one.
+template <bool IS_GL, int WG, int C, bool TWO>
 __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, const llh::BatchItem *__restrict__ items,
                           llh::BatchResult *__restrict__ out, u32 *counter) {
     Ctl &ctl = g_ctl;
@@ -366,10 +368,13 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 }
             }
             // ---- carry hand-offs: element j takes the carry out of element j-1 (wrap at 0).
-            // Round 1 moves the large carries and re-splits both digits of the pair; round 2
-            // moves the remaining small carry into the next pair's low digit.  Two rounds keep
-            // the digits near balanced even at a few bits per digit (one round lets the slack,
-            // ~sqrt(N)/4, feed back into the next squaring and grow).
+            // Round 1 moves the large carries and re-splits the pair's low digit; what it leaves
+            // for the high digit is at most about N/4 (typically sqrt(N)/4) against a balanced
+            // digit of 2^(b-1).  With floor(p/N) >= log2 N + 3 that slack is below 1/16 of a
+            // digit and stays bounded over the squarings (d' = d0 (1 + d)^2 with d0 <= 1/16), so
+            // the high digit keeps it.  With fewer bits per digit (TWO) round 1 also splits the
+            // high digit and round 2 moves that carry into the next pair's low digit, which keeps
+            // the digits balanced down to a few bits per digit.
             double *ldsd = reinterpret_cast<double *>(lds);
             const int tc = ll_opaque(t);
             __syncthreads();
@@ -385,19 +390,25 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                     const Pair q = pair_at(L);
                     double cC, cD;
                     const double x0 = split(x[s].re + cin, q.b0, cC);
-                    const double x1 = split(x[s].im + cC, q.b1, cD);
-                    x[s] = cmk(x0, x1);
-                    cout[s] = cD;
+                    if constexpr (TWO) {
+                        const double x1 = split(x[s].im + cC, q.b1, cD);
+                        x[s] = cmk(x0, x1);
+                        cout[s] = cD;
+                    } else {
+                        x[s] = cmk(x0, x[s].im + cC);
+                    }
                 }
             }
-            __syncthreads();
+            if constexpr (TWO) {
+                __syncthreads();
 #pragma unroll
-            for (int s = 0; s < R; ++s) ldsd[tc + G * s] = cout[s];
-            __syncthreads();
+                for (int s = 0; s < R; ++s) ldsd[tc + G * s] = cout[s];
+                __syncthreads();
 #pragma unroll
-            for (int s = 0; s < R; ++s) {
-                const u32 j = tc + G * s;
-                x[s].re += ldsd[j == 0 ? M - 1 : j - 1];
+                for (int s = 0; s < R; ++s) {
+                    const u32 j = tc + G * s;
+                    x[s].re += ldsd[j == 0 ? M - 1 : j - 1];
+                }
             }
         } else {
             // ---- Goldilocks: weight, NTT, square, inverse NTT
@@ -552,8 +563,10 @@ __device__ void run_tiny(const llh::ClassDesc &cd, const llh::BatchItem *__restr
 template <bool IS_GL, int WG, int C>
 __device__ __forceinline__ void run_if(const llh::ClassDesc &cd, const llh::BatchItem *__restrict__ items,
                                        llh::BatchResult *__restrict__ out, u32 *counter) {
-    if constexpr (C > 0 && kCls[C].WG == WG && (!IS_GL || kCls[C].H1 == 0))
-        run_class<IS_GL, WG, C>(cd, items, out, counter);
+    if constexpr (C > 0 && kCls[C].WG == WG && (!IS_GL || kCls[C].H1 == 0)) {
+        if (IS_GL || cd.carry2) run_class<IS_GL, WG, C, true>(cd, items, out, counter);
+        else run_class<IS_GL, WG, C, IS_GL>(cd, items, out, counter);
+    }
 }
 
 template <bool IS_GL, int WG>

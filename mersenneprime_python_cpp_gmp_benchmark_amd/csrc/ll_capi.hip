@@ -403,10 +403,14 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
         items[k].iters = w.iters;
         items[k].slot = (u32)order[k];
         items[k].pad = 0;
-        if (cls.empty() || cls.back().cls != w.cls) {
+        // FP64 carry rounds (ll_batch.hip): one from log2 N + 3 bits per digit, else two.  Within
+        // a class the items run p-descending, so each class splits into at most two entries.
+        const int two = transform == kNTT || !w.cls || w.p / w.n < (u64)(ceil_log2(w.n) + 3) ? 1 : 0;
+        if (cls.empty() || cls.back().cls != w.cls || cls.back().carry2 != two) {
             ClassDesc cd;
             std::memset(&cd, 0, sizeof(cd));
             cd.cls = w.cls;
+            cd.carry2 = two;
             const u64 M = batch_class_m(w.cls);
             cd.log_m = M ? ceil_log2(M) : 0;
             cd.log_n = w.cls ? ceil_log2(w.n) : 0;     // exact for the power-of-two classes

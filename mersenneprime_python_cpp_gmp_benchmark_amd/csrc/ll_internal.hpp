@@ -58,6 +58,7 @@ struct ClassDesc {
                         // high | inverse low | inverse high, w(e) = low[e & (S-1)] * high[e >> log_s]
     i32 log_s;          // S = 2^log_s
     i32 cls;            // size class (ll_batch.hip); 0 = the one-word lane
+    i32 carry2;         // FP64: two carry hand-off rounds (fewer than log2 N + 3 bits per digit)
 };
 constexpr int kWtabSlot = 128;
 

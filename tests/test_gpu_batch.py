@@ -142,3 +142,20 @@ def test_odd_lengths_forced_match_gmp():
             r = llhip.lucas_lehmer_n(p, k, llhip.FP64, n)
             assert r["status"] == llhip.ST_OK and r["n_used"] == n, (p, n, r)
             assert r["res64"] == want, (p, n)
+
+
+def test_carry_rounds_both_sides_of_threshold():
+    """The FP64 carry takes one hand-off round from log2 N + 3 bits per digit and two below it;
+    exponents just either side of that boundary, forced onto the length, must match GMP."""
+    from oracle import oracle as O
+    for n in (512, 4096, 5120):
+        thr = ((n - 1).bit_length() + 3) * n            # ceil(log2 N) + 3 bits per digit
+        below = [p for p in O.primes(thr - 400, thr - 1)][-2:]
+        above = [p for p in O.primes(thr, thr + 400)][:2]
+        ps = below + above
+        out = llhip.batch_n(ps, 2000, llhip.FP64, n, no_rollback=True)
+        assert (out["status"] == llhip.ST_OK).all(), (n, ps, list(out["status"]))
+        assert (out["n_used"] == n).all(), n
+        for i, p in enumerate(ps):
+            assert int(out["res64"][i]) == O.ll_gmp(p, 2000)[1], (p, n)
+        assert float(np.max(out["maxerr"])) < 0.1, (n, list(out["maxerr"]))
