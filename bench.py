@@ -14,7 +14,8 @@ exponent list; residues are created and stay on the GPU), plus
                 around the fork-join of the family launches); traffic from the committed
                 rocprofv3 PMC summary when present.
   cpu_baseline  the CPU oracle's GMP engine (restates lucaslehmer.cpp:29-56), one exponent per
-                thread (PerfNumMultiCLL.py:189-193), on a bounded sample of the same list.
+                thread (PerfNumMultiCLL.py:189-193), on a bounded sample of the same list; the
+                oracle's FP64 IBDWT engine is timed beside it (extra.cpu_fp64_engine).
   extra         the other configurations, each measured briefly: the single large exponent at
                 p = 1,048,573 (N = 2^16), p = 16,777,213 on the FP64 engine at the conservative
                 length and at the learned length, the same on the exact NTT, the round-off
@@ -90,6 +91,22 @@ def cpu_baseline(ps: list[int], every: int, threads: int) -> dict:
             "sample": f"GMP engine (restates lucaslehmer.cpp:29-56), full LL of every {every}th prime "
                       f"<= 100001 ({len(sample)} exponents, {iters} squarings, {dt:.1f} s wall), "
                       f"one exponent per thread, GMP {O.gmp_version()}"}
+
+
+def cpu_fp64_engine(ps: list[int], every: int, threads: int, iters: int = 2000) -> dict:
+    """The CPU oracle's FP64 IBDWT engine (restates lucaslehmer.cpp:256-329, with its own FFT in
+    place of FFTW's r2c/c2r: FFTW is absent from this image) at the reference's conservative
+    length, `iters` squarings of each sampled exponent."""
+    from oracle import oracle as O
+    sample = [p for p in ps[every // 2::every] if p > 2]
+    squarings = sum(min(iters, p - 2) for p in sample)
+    t0 = time.perf_counter()
+    O.ll_many(sample, iters=iters, engine=1, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(squarings / dt, 1), "unit": "LL iters/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 IBDWT engine (restates lucaslehmer.cpp:256-329; own FFT, no FFTW in the image) at "
+                      f"18 bits/word, {iters} squarings of every {every}th prime <= 100001 ({len(sample)} "
+                      f"exponents, {squarings} squarings, {dt:.1f} s wall), one exponent per thread"}
 
 
 def run_single(llhip, p: int, iters: int, transform: int, n: int = 0) -> dict:
@@ -284,6 +301,8 @@ def main() -> int:
         log("cpu baseline ...")
         cpu = cpu_baseline(ps, a.cpu_every, a.cpu_threads)
         log(f"cpu baseline: {cpu['value']:.0f} iters/s on {cpu['cores']} threads")
+        extra["cpu_fp64_engine"] = cpu_fp64_engine(ps, a.cpu_every, a.cpu_threads)
+        log(f"cpu fp64 engine: {extra['cpu_fp64_engine']['value']:.0f} iters/s")
 
     value = a.steps * total_iters / wall
     line = {
