@@ -7,9 +7,11 @@
 //  * Size classes: a class has M transform elements (FP64: N = 2M digits, GL: N = M) and is
 //    run by groups of G = M/R threads (R elements per thread, R = 8; R = 4 for M < 64).  M is a
 //    power of two (16 .. 4096) or 8 m 8 with m = 3, 5, 7 times a power of two (FP64 only).
-//  * Families: the classes of one odd factor share a workgroup size that their G divide
-//    (powers of two 512, 3: 384, 5: 320, 7: 448), one persistent kernel per family.  A
-//    workgroup holds WG/G exponents of a class at once, all stepping in lock-step.
+//  * Families: classes share a workgroup size that their G divide, one persistent kernel per
+//    family: powers of two up to M = 2048 and the one-word lane 256 (one M = 2048 exponent per
+//    workgroup, so its barriers wait on its own waves), 3 * 2^k 384, 5 * 2^k 320, 7 * 2^k 448,
+//    and M = 4096 512.  A workgroup holds WG/G exponents of a class at once, all stepping in
+//    lock-step.
 //  * Workgroups pull exponents from one queue per class (an atomic counter over items the
 //    host sorted longest-first), classes in descending length order, then the one-word lane
 //    (p < 64).  The families' launches run on separate streams, so one family's tail is
@@ -40,7 +42,7 @@ namespace {
 constexpr u32 kNone = 0xFFFFFFFFu;
 constexpr double kErrLimit = 0.4;   // round-off above this aborts the exponent (rollback longer)
 
-constexpr int kMaxGroups = 128;
+constexpr int kMaxGroups = 64;     // WG / min G (256 / 4; the other families need fewer)
 
 struct Ctl {
     u32 item[kMaxGroups];
@@ -59,15 +61,15 @@ struct Cls {
     int H1, H2;
 };
 constexpr Cls kCls[] = {
-    {0, 0, 0, 512, 0, 0, 0, 0, 0, 0},          // 0: one-word lane (p < 64)
-    {16, 4, 4, 512, 4, 4, 0, 0, 0, 0},         // 1
-    {32, 8, 4, 512, 4, 2, 4, 0, 0, 0},         // 2
-    {64, 8, 8, 512, 8, 8, 0, 0, 0, 0},         // 3
-    {128, 16, 8, 512, 8, 2, 8, 0, 0, 0},       // 4
-    {256, 32, 8, 512, 8, 4, 8, 0, 0, 0},       // 5
-    {512, 64, 8, 512, 8, 8, 8, 0, 0, 0},       // 6
-    {1024, 128, 8, 512, 8, 2, 8, 8, 0, 0},     // 7
-    {2048, 256, 8, 512, 8, 4, 8, 8, 0, 0},     // 8
+    {0, 0, 0, 256, 0, 0, 0, 0, 0, 0},          // 0: one-word lane (p < 64)
+    {16, 4, 4, 256, 4, 4, 0, 0, 0, 0},         // 1
+    {32, 8, 4, 256, 4, 2, 4, 0, 0, 0},         // 2
+    {64, 8, 8, 256, 8, 8, 0, 0, 0, 0},         // 3
+    {128, 16, 8, 256, 8, 2, 8, 0, 0, 0},       // 4
+    {256, 32, 8, 256, 8, 4, 8, 0, 0, 0},       // 5
+    {512, 64, 8, 256, 8, 8, 8, 0, 0, 0},       // 6
+    {1024, 128, 8, 256, 8, 2, 8, 8, 0, 0},     // 7
+    {2048, 256, 8, 256, 8, 4, 8, 8, 0, 0},     // 8
     {4096, 512, 8, 512, 8, 8, 8, 8, 0, 0},     // 9
     {192, 24, 8, 384, 0, 0, 0, 0, 3, 0},       // 10
     {384, 48, 8, 384, 0, 0, 0, 0, 3, 2},       // 11
@@ -84,7 +86,11 @@ constexpr Cls kCls[] = {
     {3584, 448, 8, 448, 0, 0, 0, 0, 7, 8},     // 22
 };
 constexpr int kNumCls = (int)(sizeof(kCls) / sizeof(kCls[0]));
-constexpr int kFamilyWG[4] = {512, 384, 320, 448};
+// Kernel families: 0 the powers of two up to M = 2048 and the one-word lane (one exponent of
+// M = 2048 per 4-wave workgroup, so a barrier waits on that exponent's waves only), 1 3 * 2^k,
+// 2 5 * 2^k, 3 7 * 2^k, 4 the power of two M = 4096 (G = 512).
+constexpr int kNumFam = 5;
+constexpr int kFamilyWG[kNumFam] = {256, 384, 320, 448, 512};
 
 // radix list of class C for the per-pass twiddle table (PassTable<R1 .. R5>)
 template <int C>
@@ -105,8 +111,13 @@ struct ClsRadix {
 template <int WG>
 __shared__ __attribute__((aligned(16))) char g_smem[WG * 8 * 16 * 9 / 8];
 __shared__ Ctl g_ctl;
-// FP64 two-level weight tables of the class being run (ClassDesc::wtab)
-__shared__ double g_wtab[4 * llh::kWtabSlot];
+// FP64 two-level weight tables of the class being run (ClassDesc::wtab, slots of kWtabSlot),
+// in slots of kSlot<WG> entries: the 256-thread family (N <= 4096) needs 64 per slot, which
+// keeps its LDS at 40 KB, four workgroups per CU
+template <int WG>
+constexpr int kSlot = WG == 256 ? 64 : llh::kWtabSlot;
+template <int WG>
+__shared__ double g_wtab[4 * kSlot<WG>];
 
 // Exact canonical residue of N near-balanced digits (natural order, i64, in LDS):
 // full carry propagation with wrap (2^p == 1), then Res64 = low 64 bits of the value in
@@ -196,6 +207,8 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     constexpr u32 nmask = N - 1;
     constexpr int log_s = (ilog2_ceil(N) + 1) / 2;
     constexpr u32 smask = (1u << log_s) - 1;
+    static_assert(IS_GL || ((1 << log_s) <= kSlot<WG> && ((N + smask) >> log_s) <= (u32)kSlot<WG>),
+                  "weight table slots too small for the class");
     const gptr<W> tw = uniform_table<W>(cd.tw);
     const gptr<W> twp = uniform_table<W>(cd.twp);
     const gptr<W> twpi = IS_GL ? uniform_table<W>(cd.twpi) : twp;
@@ -204,11 +217,12 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     if constexpr (!IS_GL) {
         const gptr<double> wt = uniform_table<double>(cd.wtab);
 #pragma unroll
-        for (int i = tid; i < 4 * llh::kWtabSlot; i += WG) g_wtab[i] = ld_table(wt, (u32)i);
+        for (int i = tid; i < 4 * kSlot<WG>; i += WG)
+            g_wtab<WG>[i] = ld_table(wt, (u32)(i / kSlot<WG> * llh::kWtabSlot + i % kSlot<WG>));
     }
     // weight of exponent e (forward: slots 0, 1; inverse: slots 2, 3)
     auto wlds = [&](u32 e, int inv) {
-        return g_wtab[(2 * inv) * llh::kWtabSlot + (e & smask)] * g_wtab[(2 * inv + 1) * llh::kWtabSlot + (e >> log_s)];
+        return g_wtab<WG>[(2 * inv) * kSlot<WG> + (e & smask)] * g_wtab<WG>[(2 * inv + 1) * kSlot<WG> + (e >> log_s)];
     };
     // Digits 2j, 2j + 1 of element j = t + G s, s = 0 .. R-1, in order.  With P = p mod N and
     // F = p div N (p > 64 is prime, so coprime to N, and 0 < P < N), the digit of weight
@@ -585,7 +599,7 @@ ll_batch_kernel(const llh::ClassDesc *__restrict__ classes, int ncls,
             LL_CLASS(22)
 #undef LL_CLASS
             case 0:
-                if constexpr (WG == 512) run_tiny(cd, items, out, counters + c);
+                if constexpr (WG == 256) run_tiny(cd, items, out, counters + c);
                 break;
             default: break;
         }
@@ -616,11 +630,16 @@ int batch_class_of_m(u64 M) {
 u64 batch_class_m(int c) { return c > 0 && c < kNumCls ? (u64)kCls[c].M : 0; }
 bool batch_class_pow2(int c) { return c > 0 && c < kNumCls && kCls[c].H1 == 0; }
 int batch_class_family(int c) {
-    const int wg = c >= 0 && c < kNumCls ? kCls[c].WG : 512;
-    for (int f = 0; f < 4; ++f)
+    const int wg = c >= 0 && c < kNumCls ? kCls[c].WG : kFamilyWG[0];
+    for (int f = 0; f < kNumFam; ++f)
         if (kFamilyWG[f] == wg) return f;
     return 0;
 }
+int batch_class_factor(int c) {
+    const int f = batch_class_family(c);
+    return f == 4 ? 0 : f;
+}
+int batch_num_families() { return kNumFam; }
 int batch_class_groups(int c) { return c > 0 && c < kNumCls ? kCls[c].WG / kCls[c].G : 1; }
 
 // Full-table exponent of every entry of class c's per-pass twiddle table.
@@ -636,54 +655,55 @@ bool batch_pass_exponents(int c, std::vector<u32> &ex) {
     }
 }
 
-int batch_family_wg(int family) { return kFamilyWG[family & 3]; }
+int batch_family_wg(int family) { return family >= 0 && family < kNumFam ? kFamilyWG[family] : 0; }
 
 // Launch family `family`'s persistent batch kernel on `stream`; grid = resident workgroups.
 hipError_t launch_batch(int transform, int family, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
                         BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream) {
-    const int wg = kFamilyWG[family & 3];
     const bool gl = transform == kNTT;
-    if (gl && family != 0) return hipErrorInvalidValue;
+    if (family < 0 || family >= kNumFam || (gl && family != 0 && family != 4)) return hipErrorInvalidValue;
+    const int wg = kFamilyWG[family];
+#define LL_LAUNCH(GL, W)                                                                                         \
+    hipLaunchKernelGGL((ll_batch_kernel<GL, W>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls, d_items, d_out, \
+                       d_counters)
     switch (family) {
-        case 1:
-            hipLaunchKernelGGL((ll_batch_kernel<false, 384>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls, d_items,
-                               d_out, d_counters);
-            break;
-        case 2:
-            hipLaunchKernelGGL((ll_batch_kernel<false, 320>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls, d_items,
-                               d_out, d_counters);
-            break;
-        case 3:
-            hipLaunchKernelGGL((ll_batch_kernel<false, 448>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls, d_items,
-                               d_out, d_counters);
+        case 1: LL_LAUNCH(false, 384); break;
+        case 2: LL_LAUNCH(false, 320); break;
+        case 3: LL_LAUNCH(false, 448); break;
+        case 4:
+            if (gl) LL_LAUNCH(true, 512);
+            else LL_LAUNCH(false, 512);
             break;
         default:
-            if (gl)
-                hipLaunchKernelGGL((ll_batch_kernel<true, 512>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls,
-                                   d_items, d_out, d_counters);
-            else
-                hipLaunchKernelGGL((ll_batch_kernel<false, 512>), dim3(grid), dim3(wg), 0, stream, d_classes, ncls,
-                                   d_items, d_out, d_counters);
+            if (gl) LL_LAUNCH(true, 256);
+            else LL_LAUNCH(false, 256);
             break;
     }
+#undef LL_LAUNCH
     return hipGetLastError();
 }
 
 int batch_blocks_per_cu(int transform, int family) {
     const bool gl = transform == kNTT;
-    const int wg = kFamilyWG[family & 3];
+    if (family < 0 || family >= kNumFam || (gl && family != 0 && family != 4)) return 1;
+    const int wg = kFamilyWG[family];
     int nb = 0;
     hipError_t e;
+#define LL_OCC(GL, W) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<GL, W>, wg, 0)
     switch (family) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 384>, wg, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 320>, wg, 0); break;
-        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 448>, wg, 0); break;
+        case 1: LL_OCC(false, 384); break;
+        case 2: LL_OCC(false, 320); break;
+        case 3: LL_OCC(false, 448); break;
+        case 4:
+            if (gl) LL_OCC(true, 512);
+            else LL_OCC(false, 512);
+            break;
         default:
-            e = gl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<true, 512>, wg, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ll_batch_kernel<false, 512>, wg, 0);
+            if (gl) LL_OCC(true, 256);
+            else LL_OCC(false, 256);
             break;
     }
-    if (gl && family != 0) return 1;
+#undef LL_OCC
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
 }

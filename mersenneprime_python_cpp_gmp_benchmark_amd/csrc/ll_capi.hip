@@ -55,7 +55,7 @@ struct DeviceState {
     hipStream_t fstream[4] = {nullptr, nullptr, nullptr, nullptr};   // batch families (0: stream)
     int cus = 0;
     double wclk_hz = 1e8;
-    int bpc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};                    // workgroups per CU
+    int bpc[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};              // workgroups per CU, per kernel family
     std::map<std::pair<int, int>, TableSet> tables;                   // (transform, class)
     void *d_items = nullptr, *d_out = nullptr, *d_cls = nullptr, *d_cnt = nullptr;
     size_t cap_items = 0, cap_cls = 0;
@@ -147,7 +147,7 @@ static int init_locked(int device) {
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         g_dev.wclk_hz = (double)khz * 1e3;
-    for (int f = 0; f < 4; ++f) {
+    for (int f = 0; f < batch_num_families() && f < 5; ++f) {
         g_dev.bpc[0][f] = batch_blocks_per_cu(kFP64, f);
         g_dev.bpc[1][f] = batch_blocks_per_cu(kNTT, f);
     }
@@ -323,7 +323,7 @@ static u64 batch_length(u64 p, int transform) {
     double best_cost = 0.0;
     for (const auto &e : batch_ladder(transform)) {
         const u64 N = e.first;
-        if (!((fam >> batch_class_family(e.second)) & 1)) continue;
+        if (!((fam >> batch_class_factor(e.second)) & 1)) continue;
         const bool ok = transform == kFP64 ? (double)p / (double)N <= bpw
                                            : (p + N - 1) / N <= (u64)ntt_max_bits(N);
         if (!ok) continue;
@@ -348,7 +348,7 @@ static u64 batch_length(u64 p, int transform) {
 static u64 next_batch_length(u64 p, u64 N, int transform) {
     const int fam = g_batch_families.load() | 1;
     for (const auto &e : batch_ladder(transform))
-        if (e.first > N && ((fam >> batch_class_family(e.second)) & 1)) return batch_fits(p, e.first) ? e.first : 0;
+        if (e.first > N && ((fam >> batch_class_factor(e.second)) & 1)) return batch_fits(p, e.first) ? e.first : 0;
     return 0;
 }
 
@@ -376,11 +376,11 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
     const size_t n = work.size();
     res.assign(n, BatchResult());
     if (!n) return LLH_OK;
-    // family launch order: odd-factor families (the longest exponents) first, powers of two and
-    // the one-word lane last
+    // family launch order: odd-factor families (the longest exponents) first, then the power of
+    // two M = 4096, the shorter powers of two and the one-word lane last
     auto fam_rank = [](int c) {
         const int f = batch_class_family(c);
-        return f == 0 ? 3 : f - 1;
+        return f == 0 ? 4 : f == 4 ? 3 : f - 1;
     };
     std::vector<size_t> order(n);
     for (size_t i = 0; i < n; ++i) order[i] = i;
@@ -441,12 +441,15 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
     HIPCHK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(ClassDesc), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(d_cnt, 0, cls.size() * sizeof(u32), st));
     const int tr = transform == kNTT ? 1 : 0;
-    hipEvent_t ev0, ev1, evf[4];
+    // kernel family f runs on stream f (the M = 4096 family shares the 3 * 2^k family's stream:
+    // four streams fill the four hardware queues, and the cost table never picks 3 * 2^k there)
+    constexpr int kFam = 5;
+    hipEvent_t ev0, ev1, evf[kFam];
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
-    for (int f = 0; f < 4; ++f) HIPCHK(hipEventCreateWithFlags(&evf[f], hipEventDisableTiming));
+    for (int f = 0; f < kFam; ++f) HIPCHK(hipEventCreateWithFlags(&evf[f], hipEventDisableTiming));
     HIPCHK(hipEventRecord(ev0, st));
-    bool joined[4] = {false, false, false, false};
+    bool joined[kFam] = {false, false, false, false, false};
     for (size_t i = 0; i < cls.size();) {
         // the run of class entries of one family
         size_t j = i;
@@ -457,7 +460,7 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
             ++j;
         }
         const int f = fam_of[i];
-        hipStream_t fs = g_dev.fstream[f];
+        hipStream_t fs = g_dev.fstream[f == 4 ? 1 : f];
         if (fs != st) HIPCHK(hipStreamWaitEvent(fs, ev0, 0));
         const int grid = (int)std::min<size_t>(std::max<size_t>(waves, 1), (size_t)g_dev.cus * g_dev.bpc[tr][f]);
         HIPCHK(launch_batch(transform, f, d_cls + i, (int)(j - i), d_items, d_out, d_cnt + i, grid, fs));
@@ -467,7 +470,7 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
         }
         i = j;
     }
-    for (int f = 0; f < 4; ++f)
+    for (int f = 0; f < kFam; ++f)
         if (joined[f]) HIPCHK(hipStreamWaitEvent(st, evf[f], 0));
     HIPCHK(hipEventRecord(ev1, st));
     HIPCHK(hipMemcpyAsync(res.data(), d_out, n * sizeof(BatchResult), hipMemcpyDeviceToHost, st));
@@ -476,7 +479,7 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
-    for (int f = 0; f < 4; ++f) (void)hipEventDestroy(evf[f]);
+    for (int f = 0; f < kFam; ++f) (void)hipEventDestroy(evf[f]);
     g_last_kernel_ms += ms;
     g_last_launches += 1;
     return LLH_OK;

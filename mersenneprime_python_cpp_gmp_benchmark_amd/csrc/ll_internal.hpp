@@ -67,7 +67,11 @@ int batch_num_classes();
 int batch_class_of_m(u64 M);            // class of transform length M (0 = none)
 u64 batch_class_m(int c);               // transform length of class c
 bool batch_class_pow2(int c);
-int batch_class_family(int c);          // 0: powers of two, 1: 3 * 2^k, 2: 5 * 2^k, 3: 7 * 2^k
+int batch_num_families();
+int batch_class_family(int c);          // kernel family: 0 powers of two to M = 2048, 1 3 * 2^k,
+                                        // 2 5 * 2^k, 3 7 * 2^k, 4 the power of two M = 4096
+int batch_class_factor(int c);          // length family (bit of llh_set_batch_families): 0 powers
+                                        // of two, 1 3 * 2^k, 2 5 * 2^k, 3 7 * 2^k
 int batch_class_groups(int c);          // exponents per workgroup
 int batch_family_wg(int family);
 bool batch_pass_exponents(int c, std::vector<u32> &ex);   // per-pass twiddle order of class c
