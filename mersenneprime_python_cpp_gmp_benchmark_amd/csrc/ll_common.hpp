@@ -44,6 +44,18 @@ struct DigitWalk {
     LL_HD u32 wexp(u32 r) const { return r ? n - r : 0u; }
 };
 
+// The same layout walked by weight exponent, for p coprime to N (a prime p > 7): digit d has
+// e_d = (-d p) mod N and b_d = F + [e_d < P] bits (the closed form of the ceiling definition
+// above), and e_{d+1} = (e_d - P) mod N, so each digit costs one compare and one modular
+// subtraction.
+struct ExpWalk {
+    u32 n, P, F;
+    LL_HD u32 sub(u32 e, u32 delta) const { return e < delta ? e - delta + n : e - delta; }   // e, delta < N
+    LL_HD u32 next(u32 e) const { return sub(e, P); }
+    LL_HD int bits(u32 e) const { return (int)(F + (e < P ? 1u : 0u)); }
+    LL_HD u32 of_residue(u32 r) const { return r ? n - r : 0u; }   // e_d from r = d p mod N
+};
+
 // Returns v unchanged, but opaque to the optimiser.  Index math derived from it is then
 // recomputed where it is used instead of being hoisted out of the squaring loop and kept
 // live in registers for its whole length (twiddle and weight addresses are loop-invariant,

@@ -161,6 +161,7 @@ LL_DEV u32 wexp(u64 d, u32 p, const NDiv &D) {
 LL_HD NDiv ndiv_of(const LargeArgs &A) { return NDiv{A.n, A.inv_n, A.log_n}; }
 
 LL_DEV DigitWalk walk_of(const LargeArgs &A) { return DigitWalk{(u32)A.n, A.p_mod_n, A.p_div_n}; }
+LL_DEV ExpWalk ewalk_of(const LargeArgs &A) { return ExpWalk{(u32)A.n, A.p_mod_n, A.p_div_n}; }
 // d p mod N for one position (one exact division; used once per thread or per row)
 LL_DEV u32 rmod(u64 d, u32 p, const NDiv &D) {
     u64 q, r;
@@ -168,10 +169,11 @@ LL_DEV u32 rmod(u64 d, u32 p, const NDiv &D) {
     return (u32)r;
 }
 
-// balanced split of v at b bits: v = x + c 2^b, x in [-2^(b-1), 2^(b-1))
+// balanced split of v at b bits: v = x + c 2^b, |x| <= 2^(b-1); both steps exact for integer v
+// below 2^53 (a power-of-two scaling, then the exact difference)
 LL_DEV double split_d(double v, int b, double &c) {
-    c = floor(__fma_rn(v, ldexp(1.0, -b), 0.5));
-    return __fma_rn(-c, ldexp(1.0, b), v);
+    c = __builtin_rint(ldexp(v, -b));
+    return v - ldexp(c, b);
 }
 LL_DEV i64 split_i(i64 v, int b, i64 &c) {
     c = (v + ((i64)1 << (b - 1))) >> b;
@@ -275,20 +277,20 @@ template <bool IS_GL, int W, int H>
 LL_DEV void col_weights(typename Col<IS_GL, W, H>::WT (&wt)[nwt(IS_GL)], const void *lo,
                         const void *hi, const LargeArgs &A, int w, int t) {
     typedef Col<IS_GL, W, H> C;
-    const DigitWalk DW = walk_of(A);
+    const ExpWalk EW = ewalk_of(A);
     const NDiv D = ndiv_of(A);
     const u64 j0 = (u64)w + (u64)W * t;
-    u32 r = rmod(IS_GL ? j0 : 2 * j0, A.p, D);
+    u32 e = EW.of_residue(rmod(IS_GL ? j0 : 2 * j0, A.p, D));
     const u32 step = rmod((IS_GL ? 1ull : 2ull) * W * C::GH, A.p, D);
 #pragma unroll
     for (int q = 0; q < kR; ++q) {
         if constexpr (IS_GL) {
-            wt[q] = tw2<u64>(lo, hi, DW.wexp(r), A.log_sw);
+            wt[q] = tw2<u64>(lo, hi, e, A.log_sw);
         } else {
-            wt[2 * q] = wt2d(lo, hi, DW.wexp(r), A.log_sw);
-            wt[2 * q + 1] = wt2d(lo, hi, DW.wexp(DW.next(r)), A.log_sw);
+            wt[2 * q] = wt2d(lo, hi, e, A.log_sw);
+            wt[2 * q + 1] = wt2d(lo, hi, EW.next(e), A.log_sw);
         }
-        r = DW.add(r, step);
+        e = EW.sub(e, step);
     }
 }
 
@@ -331,29 +333,29 @@ LL_DEV void col_inverse(typename Col<IS_GL, W, H>::T (&x)[kR], typename Col<IS_G
 #pragma unroll
     for (int q = 0; q < kR; ++q) lds_all[(t + C::GH * q) * kCW + c] = x[q];
     __syncthreads();
-    const DigitWalk DW = walk_of(A);
+    const ExpWalk EW = ewalk_of(A);
     const NDiv D = ndiv_of(A);
 #pragma unroll
     for (int i = 0; i < C::NROW; ++i) {
         const int h = tid + C::NT * i;
         T *row = lds_all + h * kCW;
         const u64 j0 = (u64)blk * kCW + (u64)W * h;
-        u32 r = rmod(IS_GL ? j0 : 2 * j0, A.p, D);
+        u32 e = EW.of_residue(rmod(IS_GL ? j0 : 2 * j0, A.p, D));
         CT cv = 0;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             if constexpr (IS_GL) {
                 i64 c2;
-                const i64 v = split_i((i64)row[cc] + cv, DW.bits(r), c2);
-                r = DW.next(r);
+                const i64 v = split_i((i64)row[cc] + cv, EW.bits(e), c2);
+                e = EW.next(e);
                 row[cc] = (u64)v;
                 cv = c2;
             } else {
                 double c1, c2;
-                const int b0 = DW.bits(r);
-                r = DW.next(r);
-                const int b1 = DW.bits(r);
-                r = DW.next(r);
+                const int b0 = EW.bits(e);
+                e = EW.next(e);
+                const int b1 = EW.bits(e);
+                e = EW.next(e);
                 const double x0 = split_d(row[cc].re + cv, b0, c1);
                 const double x1 = split_d(row[cc].im + c1, b1, c2);
                 row[cc] = cmk(x0, x1);
@@ -377,20 +379,20 @@ LL_DEV void col_forward(typename Col<IS_GL, W, H>::T *lds_all, TP tw,
     typedef typename C::T T;
     typedef typename C::CT CT;
     typedef typename C::S S;
-    const DigitWalk DW = walk_of(A);
+    const ExpWalk EW = ewalk_of(A);
     const NDiv D = ndiv_of(A);
 #pragma unroll
     for (int i = 0; i < C::NROW; ++i) {
         const int h = tid + C::NT * i;
         T *row = lds_all + h * kCW;
         const u64 j0 = (u64)blk * kCW + (u64)W * h;
-        u32 r = rmod(IS_GL ? j0 : 2 * j0, A.p, D);
+        u32 e = EW.of_residue(rmod(IS_GL ? j0 : 2 * j0, A.p, D));
         CT cv = cin[i];
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             if constexpr (IS_GL) {
-                const int b = DW.bits(r);
-                r = DW.next(r);
+                const int b = EW.bits(e);
+                e = EW.next(e);
                 i64 v = (i64)row[cc] + cv;
                 if (cc == kCW - 1) { row[cc] = (u64)v; break; }
                 i64 c2;
@@ -398,10 +400,10 @@ LL_DEV void col_forward(typename Col<IS_GL, W, H>::T *lds_all, TP tw,
                 row[cc] = (u64)v;
                 cv = c2;
             } else {
-                const int b0 = DW.bits(r);
-                r = DW.next(r);
-                const int b1 = DW.bits(r);
-                r = DW.next(r);
+                const int b0 = EW.bits(e);
+                e = EW.next(e);
+                const int b1 = EW.bits(e);
+                e = EW.next(e);
                 double c1, c2;
                 const double x0 = split_d(row[cc].re + cv, b0, c1);
                 double x1 = row[cc].im + c1;
