@@ -10,8 +10,11 @@ the low 64 bits of the canonical residue.  Each fixture is plain CSV:
                          p = 16,777,213 @ 5,000; p = 3,021,377 @ 20,000 = README.md:200-206)
   c4_exponents.csv       256 primes from [2^20, 2^25), numpy default_rng(0), with split
   c4_gmp_heldout.csv     held-out subset: p,iters,res64 at 20,000 iterations
+  above_batch_gmp.csv    the four primes just above 21 x 8192 (beyond the batched FP64
+                         lengths, so a batch call runs them on the large engine), full test:
+                         p,prime,res64
 
-Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp [--threads T]
+Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp|above [--threads T]
 """
 from __future__ import annotations
 
@@ -97,9 +100,20 @@ def c4_gmp(list_path: str, path: str, threads: int, count: int) -> None:
     print(f"{len(sub)} held-out residues in {time.time() - t0:.0f} s")
 
 
+def above_batch(path: str, threads: int) -> None:
+    ps = O.primes(21 * 8192 + 1, 21 * 8192 + 200)[:4]
+    t0 = time.time()
+    out = O.ll_many(ps, engine=0, threads=threads)
+    with open(path, "w") as f:
+        f.write("p,prime,res64\n")
+        for p, (v, r) in zip(ps, out):
+            f.write(f"{p},{int(v)},{r:016x}\n")
+    print(f"{len(ps)} exponents in {time.time() - t0:.0f} s")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp"])
+    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above"])
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--count", type=int, default=8)
     a = ap.parse_args()
@@ -109,6 +123,8 @@ def main() -> None:
         run_range(100001, os.path.join(HERE, "c3_gmp_p100001.csv"), a.threads)
     elif a.what == "big":
         run_big(os.path.join(HERE, "big_gmp_interim.csv"))
+    elif a.what == "above":
+        above_batch(os.path.join(HERE, "above_batch_gmp.csv"), a.threads)
     elif a.what == "c4list":
         c4_list(os.path.join(HERE, "c4_exponents.csv"))
     else:
