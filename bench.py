@@ -15,7 +15,9 @@ exponent list; residues are created and stay on the GPU), plus
                 rocprofv3 PMC summary when present.
   cpu_baseline  the CPU oracle's GMP engine (restates lucaslehmer.cpp:29-56), one exponent per
                 thread (PerfNumMultiCLL.py:189-193), on a bounded sample of the same list; the
-                oracle's FP64 IBDWT engine is timed beside it (extra.cpu_fp64_engine).
+                oracle's FP64 IBDWT engine is timed beside it (extra.cpu_fp64_engine), and the
+                single-test CPU path at p = 1,048,573 and 16,777,213 (GMP squaring, plain and
+                split over 3 and 9 threads; extra.cpu_gmp_square) beside the GPU's us/iteration.
   extra         the other configurations, each measured briefly: the single large exponent at
                 p = 1,048,573 (N = 2^16), p = 16,777,213 on the FP64 engine at the conservative
                 length and at the learned length, the same on the exact NTT, the round-off
@@ -107,6 +109,24 @@ def cpu_fp64_engine(ps: list[int], every: int, threads: int, iters: int = 2000) 
             "sample": f"FP64 IBDWT engine (restates lucaslehmer.cpp:256-329; own FFT, no FFTW in the image) at "
                       f"18 bits/word, {iters} squarings of every {every}th prime <= 100001 ({len(sample)} "
                       f"exponents, {squarings} squarings, {dt:.1f} s wall), one exponent per thread"}
+
+
+def cpu_square(bits_list, gpu_us: dict) -> dict:
+    """The reference's single-test CPU path at the large exponents: seconds per GMP squaring of a
+    p-bit number, plain and split over 3 and 9 threads (the oracle's restatement of
+    lucaslehmer.cpp:83-205, bench_square), beside the GPU's microseconds per LL iteration."""
+    from oracle import oracle as O
+    out = {}
+    for p in bits_list:
+        it = 20 if p < (1 << 22) else 3
+        us = {f"depth{d}": round(O.bench_square(p, it, d) * 1e6, 1) for d in (0, 1, 2)}
+        best = min(us.values())
+        ent = {"cpu_us_per_squaring": us, "threads": {"depth0": 1, "depth1": 3, "depth2": 9}}
+        if gpu_us.get(p):
+            ent["gpu_us_per_iter"] = gpu_us[p]
+            ent["gpu_vs_best_cpu"] = round(best / gpu_us[p], 1)
+        out[str(p)] = ent
+    return out
 
 
 def run_single(llhip, p: int, iters: int, transform: int, n: int = 0) -> dict:
@@ -303,6 +323,13 @@ def main() -> int:
         log(f"cpu baseline: {cpu['value']:.0f} iters/s on {cpu['cores']} threads")
         extra["cpu_fp64_engine"] = cpu_fp64_engine(ps, a.cpu_every, a.cpu_threads)
         log(f"cpu fp64 engine: {extra['cpu_fp64_engine']['value']:.0f} iters/s")
+        gpu_us = {}
+        if "c2_p1048573_N65536_fp64" in extra:
+            gpu_us[P20] = extra["c2_p1048573_N65536_fp64"]["us_per_iter"]
+        if "p16777213_5000_iters" in extra:
+            gpu_us[P24] = round(1e6 / extra["p16777213_5000_iters"]["plain_conservative"]["iters_per_s"], 2)
+        extra["cpu_gmp_square"] = cpu_square([P20, P24], gpu_us)
+        log(f"cpu gmp square: {extra['cpu_gmp_square']}")
 
     value = a.steps * total_iters / wall
     line = {

@@ -18,6 +18,8 @@
  *                         in place of FFTW's r2c/c2r; same weights, rounding, error, carry).
  *   orc_digits_res64      lucaslehmer.cpp:312-323 / cuda/ll_ntt.cu:238-251 (exact GMP rebuild).
  *   orc_ll_gmp_many       PerfNumMultiCLL.py:189-193 model: one exponent per thread.
+ *   orc_ll_gmp_split      lucaslehmer.cpp:83-179 (split-Karatsuba squaring on 3 or 9 threads).
+ *   orc_bench_square      lucaslehmer.cpp:181-205 (seconds per squaring, plain or split).
  *
  * Pinning (what checks this oracle, other than retyping it): the reference's known-answer
  * tests (tests/test_lucaslehmer.py), known_mp_list.txt verdicts, the README.md:200-206
@@ -29,6 +31,8 @@
 #include <gmp.h>
 #include <math.h>
 #include <pthread.h>
+#include <sched.h>
+#include <time.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -481,4 +485,190 @@ int orc_ll_many(const u64 *ps, size_t n, u64 iters, int engine, double bpw, int 
     for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
     free(th);
     return 0;
+}
+
+/* ----------------------------------------------------------------------------------------
+ * Split-Karatsuba squaring and bench_square (lucaslehmer.cpp:83-205): one test on several
+ * threads.  With s = a 2^k + b (k = half the bit length),
+ *   s^2 = a^2 2^2k + ((a+b)^2 - a^2 - b^2) 2^k + b^2,
+ * three independent half-size squarings: depth 1 runs them on three threads (two workers and
+ * the caller), depth 2 splits each again on that lane's own two workers (nine).  Below 4096
+ * bits a squaring is not split (lucaslehmer.cpp:125).  Workers spin on a generation counter,
+ * as the reference's Pool (lucaslehmer.cpp:99-124), so a split costs no thread creation.
+ * -------------------------------------------------------------------------------------- */
+typedef struct SplitPool SplitPool;
+typedef struct {
+    mpz_ptr r;
+    mpz_srcptr x;
+    SplitPool *sub;          /* depth 2: this lane's own pool; NULL = plain square */
+} SqJob;
+typedef struct {
+    mpz_t a, b, t1, t2, t3;
+} SqScratch;
+struct SplitPool {
+    pthread_t th[2];
+    SqJob job[2];
+    SqScratch scr[2];        /* scratch of the lanes the workers run (depth 2) */
+    unsigned long gen;       /* accessed with __atomic builtins */
+    int pending;
+    int stop;
+};
+
+static void square_split(mpz_ptr r, mpz_srcptr x, SqScratch *s, SplitPool *pool, SplitPool *lane_sub[3],
+                         SqScratch lane_scr[3]);
+
+static void sq_run(mpz_ptr r, mpz_srcptr x, SplitPool *sub, SqScratch *scr) {
+    if (sub) square_split(r, x, scr, sub, NULL, NULL);
+    else mpz_mul(r, x, x);
+}
+
+static void *split_worker(void *arg) {
+    void **a = (void **)arg;
+    SplitPool *P = (SplitPool *)a[0];
+    const int i = (int)(intptr_t)a[1];
+    free(arg);
+    unsigned long seen = 0;
+    for (;;) {
+        unsigned spins = 0;
+        while (__atomic_load_n(&P->gen, __ATOMIC_ACQUIRE) == seen && !__atomic_load_n(&P->stop, __ATOMIC_RELAXED)) {
+            if (++spins < 4096) __builtin_ia32_pause();
+            else { sched_yield(); spins = 0; }
+        }
+        if (__atomic_load_n(&P->stop, __ATOMIC_RELAXED)) return NULL;
+        seen = __atomic_load_n(&P->gen, __ATOMIC_ACQUIRE);
+        sq_run(P->job[i].r, P->job[i].x, P->job[i].sub, &P->scr[i]);
+        __atomic_fetch_sub(&P->pending, 1, __ATOMIC_ACQ_REL);
+    }
+}
+
+static SplitPool *pool_new(void) {
+    SplitPool *P = (SplitPool *)calloc(1, sizeof(SplitPool));
+    for (int i = 0; i < 2; ++i) {
+        mpz_init(P->scr[i].a); mpz_init(P->scr[i].b); mpz_init(P->scr[i].t1); mpz_init(P->scr[i].t2);
+        mpz_init(P->scr[i].t3);
+        void **arg = (void **)malloc(2 * sizeof(void *));
+        arg[0] = P;
+        arg[1] = (void *)(intptr_t)i;
+        pthread_create(&P->th[i], NULL, split_worker, arg);
+    }
+    return P;
+}
+
+static void pool_free(SplitPool *P) {
+    if (!P) return;
+    __atomic_store_n(&P->stop, 1, __ATOMIC_RELEASE);
+    for (int i = 0; i < 2; ++i) pthread_join(P->th[i], NULL);
+    for (int i = 0; i < 2; ++i) {
+        mpz_clear(P->scr[i].a); mpz_clear(P->scr[i].b); mpz_clear(P->scr[i].t1); mpz_clear(P->scr[i].t2);
+        mpz_clear(P->scr[i].t3);
+    }
+    free(P);
+}
+
+static void pool_wait(SplitPool *P) {
+    unsigned spins = 0;
+    while (__atomic_load_n(&P->pending, __ATOMIC_ACQUIRE) != 0) {
+        if (++spins < 4096) __builtin_ia32_pause();
+        else { sched_yield(); spins = 0; }
+    }
+}
+
+/* r = x^2; pool != NULL splits on it (its two workers take a^2 and b^2, the caller (a+b)^2).
+ * lane_sub / lane_scr (depth 2): the three lanes' own pools and scratch. */
+static void square_split(mpz_ptr r, mpz_srcptr x, SqScratch *s, SplitPool *pool, SplitPool *lane_sub[3],
+                         SqScratch lane_scr[3]) {
+    const size_t L = mpz_sizeinbase(x, 2), k = L / 2;
+    if (!pool || L < 4096) { mpz_mul(r, x, x); return; }
+    mpz_tdiv_q_2exp(s->a, x, k);
+    mpz_tdiv_r_2exp(s->b, x, k);
+    mpz_add(s->t3, s->a, s->b);
+    pool->job[0] = (SqJob){s->t1, s->a, lane_sub ? lane_sub[1] : NULL};
+    pool->job[1] = (SqJob){s->t2, s->b, lane_sub ? lane_sub[2] : NULL};
+    __atomic_store_n(&pool->pending, 2, __ATOMIC_RELEASE);
+    __atomic_fetch_add(&pool->gen, 1, __ATOMIC_ACQ_REL);
+    if (lane_sub) square_split(s->t3, s->t3, &lane_scr[0], lane_sub[0], NULL, NULL);
+    else mpz_mul(s->t3, s->t3, s->t3);
+    pool_wait(pool);
+    mpz_sub(s->t3, s->t3, s->t1);
+    mpz_sub(s->t3, s->t3, s->t2);
+    mpz_mul_2exp(r, s->t1, 2 * k);
+    mpz_mul_2exp(s->t3, s->t3, k);
+    mpz_add(r, r, s->t3);
+    mpz_add(r, r, s->t2);
+}
+
+typedef struct {
+    SplitPool *top, *lane[3];
+    SqScratch top_scr, lane_scr[3];
+} SplitCtx;
+
+static void split_ctx_init(SplitCtx *c, int depth) {
+    memset(c, 0, sizeof(*c));
+    SqScratch *all[4] = {&c->top_scr, &c->lane_scr[0], &c->lane_scr[1], &c->lane_scr[2]};
+    for (int i = 0; i < 4; ++i) {
+        mpz_init(all[i]->a); mpz_init(all[i]->b); mpz_init(all[i]->t1); mpz_init(all[i]->t2); mpz_init(all[i]->t3);
+    }
+    if (depth >= 1) c->top = pool_new();
+    if (depth >= 2)
+        for (int i = 0; i < 3; ++i) c->lane[i] = pool_new();
+}
+
+static void split_ctx_free(SplitCtx *c) {
+    pool_free(c->top);
+    for (int i = 0; i < 3; ++i) pool_free(c->lane[i]);
+    SqScratch *all[4] = {&c->top_scr, &c->lane_scr[0], &c->lane_scr[1], &c->lane_scr[2]};
+    for (int i = 0; i < 4; ++i) {
+        mpz_clear(all[i]->a); mpz_clear(all[i]->b); mpz_clear(all[i]->t1); mpz_clear(all[i]->t2); mpz_clear(all[i]->t3);
+    }
+}
+
+static void split_square(SplitCtx *c, mpz_ptr r, mpz_srcptr x) {
+    square_split(r, x, &c->top_scr, c->top, c->lane[0] ? c->lane : NULL, c->lane_scr);
+}
+
+/* lucaslehmer.cpp:147-179: the LL loop with split squarings (depth 0 = orc_ll_gmp's loop). */
+int orc_ll_gmp_split(u64 p, u64 max_iters, int depth, u64 *res64) {
+    if (p < 2) { *res64 = 0; return 0; }
+    if (p == 2) { *res64 = 0; return 1; }
+    if (p % 2 == 0) { *res64 = 0; return 0; }
+    SplitCtx c;
+    split_ctx_init(&c, depth);
+    mpz_t M, s, sq, hi;
+    mpz_init(M); mpz_init(s); mpz_init(sq); mpz_init(hi);
+    mpz_set_ui(M, 1); mpz_mul_2exp(M, M, p); mpz_sub_ui(M, M, 1);
+    mpz_set_ui(s, 4);
+    for (u64 i = 0; i + 2 < p && (!max_iters || i < max_iters); ++i) {
+        split_square(&c, sq, s);
+        mpz_sub_ui(sq, sq, 2);
+        if (mpz_sgn(sq) < 0) mpz_add(sq, sq, M);
+        mpz_tdiv_q_2exp(hi, sq, p);
+        mpz_tdiv_r_2exp(s, sq, p);
+        mpz_add(s, s, hi);
+        if (mpz_cmp(s, M) >= 0) mpz_sub(s, s, M);
+    }
+    *res64 = low64_of(s);
+    const int prime = mpz_cmp_ui(s, 0) == 0;
+    mpz_clear(M); mpz_clear(s); mpz_clear(sq); mpz_clear(hi);
+    split_ctx_free(&c);
+    return prime;
+}
+
+/* lucaslehmer.cpp:181-205: seconds per squaring of a random `bits`-bit number (depth 0 plain,
+ * 1 and 2 split), the clock around the squarings only. */
+double orc_bench_square(u64 bits, int iters, int depth) {
+    SplitCtx c;
+    split_ctx_init(&c, depth);
+    mpz_t x, r;
+    mpz_init(x); mpz_init(r);
+    gmp_randstate_t st;
+    gmp_randinit_default(st);
+    mpz_urandomb(x, st, bits);
+    gmp_randclear(st);
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int i = 0; i < iters; ++i) split_square(&c, r, x);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    mpz_clear(x); mpz_clear(r);
+    split_ctx_free(&c);
+    return ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) / (iters > 0 ? iters : 1);
 }

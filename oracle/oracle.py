@@ -70,6 +70,10 @@ def lib():
         L.orc_ll_many.argtypes = [P64, ctypes.c_size_t, _u64, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_int, ctypes.POINTER(ctypes.c_int), P64]
         L.orc_ll_many.restype = ctypes.c_int
+        L.orc_ll_gmp_split.argtypes = [_u64, _u64, ctypes.c_int, P64]
+        L.orc_ll_gmp_split.restype = ctypes.c_int
+        L.orc_bench_square.argtypes = [_u64, ctypes.c_int, ctypes.c_int]
+        L.orc_bench_square.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -183,6 +187,20 @@ def ll_many(ps, iters: int = 0, engine: int = 0, threads: int = 1, bpw: float = 
     res = (_u64 * n)()
     lib().orc_ll_many(arr, n, iters, engine, bpw, threads, ver, res)
     return [(bool(ver[i]), int(res[i])) for i in range(n)]
+
+
+def ll_gmp_split(p: int, iters: int = 0, depth: int = 1) -> tuple[bool, int]:
+    """(verdict, res64) with split-Karatsuba squarings on 3 (depth 1) or 9 (depth 2) threads
+    (lucaslehmer.cpp:83-179)."""
+    r = _u64(0)
+    v = lib().orc_ll_gmp_split(p, iters, depth, ctypes.byref(r))
+    return bool(v), int(r.value)
+
+
+def bench_square(bits: int, iters: int = 20, depth: int = 0) -> float:
+    """Seconds per GMP squaring of a random `bits`-bit number, plain or split
+    (lucaslehmer.cpp:181-205)."""
+    return float(lib().orc_bench_square(bits, iters, depth))
 
 
 def py_ll(p: int, iters: int = 0) -> tuple[bool, int]:

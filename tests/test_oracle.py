@@ -139,6 +139,20 @@ def test_digit_bits_closed_form():
                     assert we[d] == e, (n, p, d)
 
 
+def test_split_gmp_engine():
+    """The split-Karatsuba engine (lucaslehmer.cpp:83-179) finds exactly the known Mersenne
+    exponents up to 4423 at depths 1 and 2 (tests/test_lucaslehmer.py:26-29) and gives the plain
+    GMP Res64 on composites above the 4096-bit split threshold, also after a partial run."""
+    ps = O.primes(2, 4423)
+    for d in (1, 2):
+        assert [p for p in ps if p > 4000 and O.ll_gmp_split(p, 0, d)[0]] == [4253, 4423]
+        assert not O.ll_gmp_split(4425, 0, d)[0]
+        for p in (9689, 9973, 21701):
+            assert O.ll_gmp_split(p, 0, d) == O.ll_gmp(p)[:2], (p, d)
+        assert O.ll_gmp_split(44497, 300, d) == O.ll_gmp(44497, 300)[:2]
+    assert O.bench_square(1 << 16, 5, 1) > 0.0
+
+
 def test_size_rules():
     # lucaslehmer.cpp:244-254 at the configurations' exponents
     assert O.fft_length(1048573) == 65536
