@@ -30,6 +30,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import multiprocessing
 import os
 import sys
 import tempfile
@@ -261,8 +262,12 @@ def main(argv=None, engine_fn=None) -> int:
         frontier = 0
         block, block_born = [], None
         submit_order = exps_sorted if ordered else sorted(exps, reverse=True)
-        Ex = ThreadPoolExecutor if a.workers == "threads" else ProcessPoolExecutor
-        with Ex(max_workers=a.threads) as ex:
+        if a.workers == "threads":
+            pool = ThreadPoolExecutor(max_workers=a.threads)
+        else:
+            # fresh interpreters: a forked copy of a process that has used the device is not safe
+            pool = ProcessPoolExecutor(max_workers=a.threads, mp_context=multiprocessing.get_context("spawn"))
+        with pool as ex:
             if a.workers == "threads":
                 futs = [ex.submit(test_one, ll, engine, p) for p in submit_order]
             else:
