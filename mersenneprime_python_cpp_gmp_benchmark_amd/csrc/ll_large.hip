@@ -153,14 +153,8 @@ LL_DEV u64 cbit(u64 d, u32 p, const NDiv &D) {
     return q + (r != 0);
 }
 LL_DEV int bits_of(u64 d, u32 p, const NDiv &D) { return (int)(cbit(d + 1, p, D) - cbit(d, p, D)); }
-LL_DEV u32 wexp(u64 d, u32 p, const NDiv &D) {
-    u64 q, r;
-    divmod_n(d * p, D, q, r);
-    return r ? (u32)(D.n - r) : 0u;
-}
 LL_HD NDiv ndiv_of(const LargeArgs &A) { return NDiv{A.n, A.inv_n, A.log_n}; }
 
-LL_DEV DigitWalk walk_of(const LargeArgs &A) { return DigitWalk{(u32)A.n, A.p_mod_n, A.p_div_n}; }
 LL_DEV ExpWalk ewalk_of(const LargeArgs &A) { return ExpWalk{(u32)A.n, A.p_mod_n, A.p_div_n}; }
 // d p mod N for one position (one exact division; used once per thread or per row)
 LL_DEV u32 rmod(u64 d, u32 p, const NDiv &D) {
