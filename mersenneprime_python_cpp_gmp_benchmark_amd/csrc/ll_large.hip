@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fused(LargeArgs A) {
 #pragma unroll
     for (int i = 0; i < C::NROW; ++i) st_coh(carry + blk * H + tid + C::NT * i, cy[i]);
     publish_flag(A.flags + blk, seq, tid);
-    if constexpr (!IS_GL) err_publish(&s_err, eloc, A, tid);
+    if constexpr (!IS_GL) atomicMax(&s_err, (unsigned long long)__double_as_longlong(eloc));
     const int prev = blk > 0 ? blk - 1 : NBLK - 1;
     wait_flag(A.flags + prev, seq, A.fault, tid);
     CT cin[C::NROW];
@@ -580,6 +580,10 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fused(LargeArgs A) {
         ld_coh(carry + src, cin[i]);
     }
     col_forward<IS_GL, W, H>(lds_all, (const T *)s_tw, wf, tm, A, blk, c, t, tid, cin);
+    // the workgroup's round-off maximum (s_err, complete after the barriers of col_forward) goes
+    // out last, off the path of the carry hand-off, with no barrier of its own
+    if constexpr (!IS_GL)
+        if (tid == 0) atomicMax(A.err + A.err_slot, s_err);
 }
 
 template <class F, int W, int GW, bool INV, class TP>
