@@ -198,6 +198,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     static_assert(NG <= kMaxGroups, "too many groups per workgroup");
     static_assert(NG * REG * (int)sizeof(T) <= (int)sizeof(g_smem<WG>), "LDS buffer too small");
     constexpr u32 N = IS_GL ? (u32)M : 2u * (u32)M;
+    static_assert(G * R == M, "a group's threads hold the whole residue");
 
     const int tid = threadIdx.x, g = tid / G, t = tid % G;
     T *lds = reinterpret_cast<T *>(g_smem<WG>) + g * REG;
@@ -215,14 +216,24 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     const gptr<WT> wf = uniform_table<WT>(cd.wf);
     const gptr<WT> wi = uniform_table<WT>(cd.wi);
     if constexpr (!IS_GL) {
+        // The low slots (0, 2) are stored as their S/2 even entries, then the S/2 odd ones.
+        // A digit pair's exponents are e0 = (-2 j p) mod N (even, as N is even) and
+        // e1 = e0 - p mod N (odd), and across the lanes of a wave e0 steps by -2 P mod N.  In
+        // natural order the low indices e0 mod S then fall on every other bank pair (two-way
+        // conflicts on each read); in the halves the index (e0 mod S) / 2 steps by the odd P
+        // and every lane of a pass has a bank pair of its own.
         const gptr<double> wt = uniform_table<double>(cd.wtab);
 #pragma unroll
-        for (int i = tid; i < 4 * kSlot<WG>; i += WG)
-            g_wtab<WG>[i] = ld_table(wt, (u32)(i / kSlot<WG> * llh::kWtabSlot + i % kSlot<WG>));
+        for (int i = tid; i < 4 * kSlot<WG>; i += WG) {
+            const u32 slot = (u32)i / kSlot<WG>, pos = (u32)i % kSlot<WG>;
+            const u32 k = slot % 2 != 0 || pos > smask ? pos : pos <= (smask >> 1) ? 2 * pos : 2 * pos - smask;
+            g_wtab<WG>[i] = ld_table(wt, slot * llh::kWtabSlot + k);
+        }
     }
-    // weight of exponent e (forward: slots 0, 1; inverse: slots 2, 3)
-    auto wlds = [&](u32 e, int inv) {
-        return g_wtab<WG>[(2 * inv) * kSlot<WG> + (e & smask)] * g_wtab<WG>[(2 * inv + 1) * kSlot<WG> + (e >> log_s)];
+    // weight of exponent e of parity ODD (forward: slots 0, 1; inverse: slots 2, 3)
+    auto wlds = [&](u32 e, int inv, int odd) {
+        const u32 lo = (u32)odd * ((smask + 1) >> 1) + ((e >> 1) & (smask >> 1));
+        return g_wtab<WG>[(2 * inv) * kSlot<WG> + lo] * g_wtab<WG>[(2 * inv + 1) * kSlot<WG> + (e >> log_s)];
     };
     // Digits 2j, 2j + 1 of element j = t + G s, s = 0 .. R-1, in order.  With P = p mod N and
     // F = p div N (p > 64 is prime, so coprime to N, and 0 < P < N), the digit of weight
@@ -233,14 +244,20 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
         u32 P, F, step, e;
     };
     auto sub_mod = [](u32 a, u32 b) -> u32 {      // (a - b) mod N for a, b < N
-        if constexpr (P2) return (a - b) & nmask;
-        else return a < b ? a - b + N : a - b;
+        if constexpr (P2) {
+            return (a - b) & nmask;
+        } else {
+            // for a < b the difference wraps to at least 2^32 - N and d + N is the residue;
+            // otherwise d < N <= d + N: the smaller of the two is (a - b) mod N
+            const u32 d = a - b;
+            return min(d, d + N);
+        }
     };
     auto layout = [&](u32 pp, int tt) {
         Layout L;
         L.P = pp % N;
         L.F = pp / N;
-        L.step = (2u * (u32)G * L.P) % N;
+        L.step = 2u * (u32)G * (L.P % (u32)R);     // (2 G P) mod N, as N = 2 G R
         L.e = sub_mod(0u, (2u * (u32)tt * L.P) % N);
         return L;
     };
@@ -331,24 +348,27 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const Pair q = pair_at(L);
-                    x[s] = cmk(x[s].re * wlds(q.e0, 0), x[s].im * wlds(q.e1, 0));
+                    x[s] = cmk(x[s].re * wlds(q.e0, 0, 0), x[s].im * wlds(q.e1, 0, 1));
                 }
             }
             if constexpr (P2)
                 transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, false, true, PS>(x, lds, t, twp);
             else
                 hybrid_transform<F, M, G, R, false, K.H1, K.H2, 0, true, PS>(x, lds, t, twp);
-            // ---- square the real spectrum in packed form (pairs k, M-k)
+            // ---- square the real spectrum in packed form (pairs k, M-k).  The exchange is
+            // stored unpadded: the stores and the mirrored loads are contiguous across the
+            // lanes, which the LDS serves without bank conflicts (a pad element per 16 made
+            // every 16-byte load of it take two passes).
             const int tq = ll_opaque(t);
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < R; ++q) lds[lds_pad(tq + G * q)] = x[q];
+            for (int q = 0; q < R; ++q) lds[tq + G * q] = x[q];
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int k = tq + G * q;
                 const int km = k == 0 ? 0 : M - k;
-                const C64 zm = lds[lds_pad(km)];
+                const C64 zm = lds[km];
                 const C64 D = csub(x[q], cconj(zm));
                 const C64 w = ld_table(tw, (u32)k);
                 const C64 A = cmk(1.0 + w.re, w.im);
@@ -369,7 +389,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 for (int s = 0; s < R; ++s) {
                     const u32 j = tu + G * s;
                     const Pair q = pair_at(L);
-                    const double v0 = x[s].re * wlds(q.e0, 1), v1 = x[s].im * wlds(q.e1, 1);
+                    const double v0 = x[s].re * wlds(q.e0, 1, 0), v1 = x[s].im * wlds(q.e1, 1, 1);
                     double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
                     eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
                     if (j == 0 && has) r0 -= 2.0;

@@ -50,7 +50,12 @@ struct DigitWalk {
 // subtraction.
 struct ExpWalk {
     u32 n, P, F;
-    LL_HD u32 sub(u32 e, u32 delta) const { return e < delta ? e - delta + n : e - delta; }   // e, delta < N
+    // (e - delta) mod N for e, delta < N: with e < delta the difference wraps to at least
+    // 2^32 - N and adding N gives the residue; otherwise the difference is below N
+    LL_HD u32 sub(u32 e, u32 delta) const {
+        const u32 d = e - delta, w = d + n;
+        return w < d ? w : d;
+    }
     LL_HD u32 next(u32 e) const { return sub(e, P); }
     LL_HD int bits(u32 e) const { return (int)(F + (e < P ? 1u : 0u)); }
     LL_HD u32 of_residue(u32 r) const { return r ? n - r : 0u; }   // e_d from r = d p mod N

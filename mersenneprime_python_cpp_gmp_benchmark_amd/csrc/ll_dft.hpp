@@ -11,15 +11,14 @@
 // With r_first = r_last = R, thread t's inputs to the first pass are elements
 // t + G s (s < R) and its outputs of the last pass are elements t + G q: both are the
 // same "column" layout, so the digits stay in registers between iterations and only the
-// middle hand-offs go through LDS.  LDS indices are padded by one element per sixteen so
-// the stride-16 stores of a radix-16 pass do not pile onto one bank.
+// middle hand-offs go through LDS.  LDS indices are padded by one element per 2^PS so the
+// stride-2^PS stores of a radix-2^PS pass do not pile onto one bank.
 #pragma once
 #include <type_traits>
 #include "ll_common.hpp"
 
-LL_HD constexpr int lds_pad(int i) { return i + (i >> 4); }
 // One pad element per 2^PS: the stride-2^PS stores of a radix-2^PS first pass land on distinct
-// banks for every 8 consecutive lanes (16-byte elements).  PS = 4 is lds_pad.
+// banks for every 8 consecutive lanes (16-byte stores are served 8 lanes at a time, 128 bytes).
 template <int PS>
 LL_HD constexpr int lds_pad_s(int i) { return i + (i >> PS); }
 

@@ -633,7 +633,7 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
         const int tt = ll_opaque(t);
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < kR; ++q) lds[lds_pad(tt + GW * q)] = x[q];
+        for (int q = 0; q < kR; ++q) lds[tt + GW * q] = x[q];     // unpadded: contiguous stores, mirrored loads
         __syncthreads();
         // partner of (row, k_w): row 0 -> (0, -k_w); row H/2 -> (H/2, W-1-k_w);
         // otherwise -> (H - row, W-1-k_w), held by the other group of this workgroup
@@ -642,7 +642,7 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
         for (int q = 0; q < kR; ++q) {
             const int kw = tt + GW * q;
             const int kwp = row == 0 ? (kw == 0 ? 0 : W - kw) : (W - 1 - kw);
-            const C64 zm = plds[lds_pad(kwp)];
+            const C64 zm = plds[kwp];
             const C64 D = csub(x[q], cconj(zm));
             const C64 AD = cmul(cmk(1.0 + wk[q].re, wk[q].im), csqr(D));
             const C64 Z2 = csqr(x[q]);
