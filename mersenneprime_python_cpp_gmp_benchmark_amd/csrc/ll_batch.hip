@@ -192,6 +192,10 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
     typedef typename std::conditional<IS_GL, u64, double>::type WT;
     constexpr int NG = WG / G;
     constexpr int PS = R == 8 ? 3 : 2;            // LDS pad shift: one pad element per R
+    // FP64 hand-offs after the first use one pad element per 32: their 16-byte stores are runs of
+    // consecutive elements, and the LDS serves contiguous 16-byte loads of a wave without bank
+    // conflicts, where one pad per 8 (needed by the first pass's stride-8 stores) costs two passes
+    constexpr int PSM = R == 8 ? 5 : PS;
     static_assert(R == 8 || R == 4, "per-thread counts 4 and 8");
     static_assert(WG % G == 0, "groups must tile the workgroup");
     constexpr int REG = M + (M >> PS);
@@ -352,7 +356,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 }
             }
             if constexpr (P2)
-                transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, false, true, PS>(x, lds, t, twp);
+                transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, false, true, PS, PSM>(x, lds, t, twp);
             else
                 hybrid_transform<F, M, G, R, false, K.H1, K.H2, 0, true, PS>(x, lds, t, twp);
             // ---- square the real spectrum in packed form (pairs k, M-k).  The exchange is
@@ -377,7 +381,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
             }
             if constexpr (P2)
-                transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, true, true, PS>(x, lds, t, twp);
+                transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, true, true, PS, PSM>(x, lds, t, twp);
             else
                 hybrid_transform<F, M, G, R, true, K.H1, K.H2, 0, true, PS>(x, lds, t, twp);
             // ---- unweight, round, error, -2, carry within the pair

@@ -213,8 +213,9 @@ LL_DEV void dft_r(typename F::T *x) {
 // Addressing: G is a multiple of L (L <= M / R = G) so k = (t + G b) mod L = t mod L, and the
 // padded LDS index of every access is a per-thread base plus a compile-time constant (an
 // immediate offset): for C a multiple of P, pad(t + C) = pad(t) + C (P + 1) / P.
+// PS pads the loads, PSW the stores (the next pass loads with PSW as its PS).
 template <class F, int M, int G, int R, int r, int L, bool FIRST, bool LAST, bool INV, int OFF = -1, int PS = 4,
-          class TP>
+          int PSW = PS, class TP>
 LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     static_assert(R % r == 0, "radix must divide the per-thread count");
     static_assert(!FIRST || r == R, "first pass must use the full per-thread radix");
@@ -254,48 +255,62 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
 #pragma unroll
     for (int b = 0; b < NB; ++b) dft_r<F, r, INV>(&x[b * r]);
     if constexpr (!LAST) {
+        constexpr int PW = 1 << PSW;
         __syncthreads();
         // output q of butterfly i = t + G b goes to (i - k) r + k + q L
-        if constexpr (L == 1 && r == P) {
-            // i r + q, q < P: padded i (r + 1) + q
+        if constexpr (L == 1 && r == PW) {
+            // i r + q, q < PW: padded i (r + 1) + q
             const int wb = t * (r + 1);
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int q = 0; q < r; ++q) lds[wb + G * b * (r + 1) + q] = x[b * r + q];
-        } else if constexpr (L % P == 0) {
-            // (t - k) r, G b r and q L are multiples of P; k < L adds k + k / P
+        } else if constexpr (L % PW == 0) {
+            // (t - k) r, G b r and q L are multiples of PW; k < L adds k + k / PW
             const int k = (int)((u32)t % (u32)L);
-            const int wb = (int)((u32)(t - k) * (u32)r / (u32)P * (u32)(P + 1)) + k + (k >> PS);
+            const int wb = (int)((u32)(t - k) * (u32)r / (u32)PW * (u32)(PW + 1)) + k + (k >> PSW);
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
-                for (int q = 0; q < r; ++q) lds[wb + (G * b * r + q * L) / P * (P + 1)] = x[b * r + q];
+                for (int q = 0; q < r; ++q) lds[wb + (G * b * r + q * L) / PW * (PW + 1)] = x[b * r + q];
+        } else if constexpr (PW % L == 0 && (r * L) % PW == 0) {
+            // (t - k) r and G b r are multiples of r L, so of PW; k + q L stays inside one
+            // PW-aligned block (L divides PW), which adds q L / PW pad elements
+            const int k = (int)((u32)t % (u32)L);
+            const int wb = (int)((u32)(t - k) * (u32)r / (u32)PW * (u32)(PW + 1)) + k;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < r; ++q) lds[wb + G * b * r / PW * (PW + 1) + q * L + q * L / PW] = x[b * r + q];
         } else {
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int i = t + G * b;
                 const int k = i % L;
 #pragma unroll
-                for (int q = 0; q < r; ++q) lds[lds_pad_s<PS>((i - k) * r + k + q * L)] = x[b * r + q];
+                for (int q = 0; q < r; ++q) lds[lds_pad_s<PSW>((i - k) * r + k + q * L)] = x[b * r + q];
             }
         }
     }
 }
 
 // Whole transform: two, three or four passes (R3 == 0: two; R4 == 0: three).  PP: `tw` is the
-// per-pass table of PassTable<R1, R2, R3, R4, 0>; PS: LDS pad shift.
+// per-pass table of PassTable<R1, R2, R3, R4, 0>; PS: LDS pad shift of the first hand-off, whose
+// stores are strided by R1.  PSM: pad shift of the later hand-offs, whose stores are runs of L
+// consecutive elements; it is used where both of their stores keep constant offsets, else PS.
 template <class F, int M, int G, int R, int R1, int R2, int R3, int R4, bool INV, bool PP = false, int PS = 4,
-          class TP>
+          int PSM = PS, class TP>
 LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     typedef PassTable<R1, R2, R3, R4, 0> PT;
     constexpr int O2 = PP ? PT::O2 : -1, O3 = PP ? PT::O3 : -1, O4 = PP ? PT::O4 : -1;
+    constexpr int PM = 1 << PSM;
+    constexpr int QS = ((R1 * R2) % PM == 0 && PM % R1 == 0 && G % PM == 0) ? PSM : PS;
     if constexpr (R4 != 0) {
         static_assert(R1 * R2 * R3 * R4 == M, "radices must multiply to M");
         stockham_pass<F, M, G, R, R1, 1, true, false, INV, -1, PS>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R2, R1, false, false, INV, O2, PS>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R3, R1 * R2, false, false, INV, O3, PS>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R4, R1 * R2 * R3, false, true, INV, O4, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, false, INV, O2, PS, QS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R3, R1 * R2, false, false, INV, O3, QS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R4, R1 * R2 * R3, false, true, INV, O4, QS>(x, lds, t, tw);
     } else if constexpr (R3 == 0) {
         static_assert(R1 * R2 == M, "radices must multiply to M");
         stockham_pass<F, M, G, R, R1, 1, true, false, INV, -1, PS>(x, lds, t, tw);
@@ -303,8 +318,8 @@ LL_DEV void transform(typename F::T (&x)[R], typename F::T *lds, int t, TP tw) {
     } else {
         static_assert(R1 * R2 * R3 == M, "radices must multiply to M");
         stockham_pass<F, M, G, R, R1, 1, true, false, INV, -1, PS>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R2, R1, false, false, INV, O2, PS>(x, lds, t, tw);
-        stockham_pass<F, M, G, R, R3, R1 * R2, false, true, INV, O3, PS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R2, R1, false, false, INV, O2, PS, QS>(x, lds, t, tw);
+        stockham_pass<F, M, G, R, R3, R1 * R2, false, true, INV, O3, QS>(x, lds, t, tw);
     }
 }
 
