@@ -225,6 +225,20 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
     constexpr int STRIDE = M / r;
     constexpr int P = 1 << PS;
     t = ll_opaque(t);
+    // The twiddles depend on k = t mod L only (G is a multiple of L), so they are the same for
+    // every butterfly of the thread.  They are loaded before the barrier: the table is
+    // read-only, and the loads stay in flight while the workgroup meets and reads the LDS.
+    typedef typename F::W W;
+    W wv[L > 1 ? r - 1 : 1];
+    if constexpr (L > 1) {
+        const int k = (int)((u32)t % (u32)L);     // t >= 0: unsigned remainder
+        constexpr int TSTEP = M / (r * L);
+#pragma unroll
+        for (int s = 1; s < r; ++s) {
+            const u32 e = OFF >= 0 ? (u32)(OFF + (s - 1) * L + k) : (u32)(s * k * TSTEP);
+            wv[s - 1] = tw_at<W>(tw, e);
+        }
+    }
     if constexpr (!FIRST) {
         __syncthreads();
         if constexpr (G % P == 0 && STRIDE % P == 0) {
@@ -241,15 +255,10 @@ LL_DEV void stockham_pass(typename F::T (&x)[R], typename F::T *lds, int t, TP t
         }
     }
     if constexpr (L > 1) {
-        const int k = (int)((u32)t % (u32)L);     // t >= 0: unsigned remainder
-        constexpr int TSTEP = M / (r * L);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
 #pragma unroll
-            for (int s = 1; s < r; ++s) {
-                const u32 e = OFF >= 0 ? (u32)(OFF + (s - 1) * L + k) : (u32)(s * k * TSTEP);
-                x[b * r + s] = F::template twiddle<INV>(x[b * r + s], tw_at<typename F::W>(tw, e));
-            }
+            for (int s = 1; s < r; ++s) x[b * r + s] = F::template twiddle<INV>(x[b * r + s], wv[s - 1]);
         }
     }
 #pragma unroll
@@ -342,6 +351,18 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
     T y[NB][r];
     t = ll_opaque(t);
     const int kt = (int)((u32)t % (u32)L);
+    // FAST: every butterfly of the thread has k = kt, so its twiddles are loaded once, before
+    // the barrier (read-only table; the loads stay in flight across it)
+    typedef typename F::W W;
+    constexpr bool PRE = FAST && L > 1;
+    W wv[PRE ? r - 1 : 1];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int s = 1; s < r; ++s) {
+            const u32 e = OFF >= 0 ? (u32)(OFF + (s - 1) * L + kt) : (u32)(s * kt * TSTEP);
+            wv[s - 1] = tw_at<W>(tw, e);
+        }
+    }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -356,11 +377,14 @@ LL_DEV void mixed_pass(typename F::T *lds, int t, TP tw) {
 #pragma unroll
                 for (int s = 0; s < r; ++s) y[b][s] = lds[lds_pad_s<PS>(i + s * NBF)];
             }
-            if constexpr (L > 1) {
+            if constexpr (PRE) {
+#pragma unroll
+                for (int s = 1; s < r; ++s) y[b][s] = F::template twiddle<INV>(y[b][s], wv[s - 1]);
+            } else if constexpr (L > 1) {
 #pragma unroll
                 for (int s = 1; s < r; ++s) {
                     const u32 e = OFF >= 0 ? (u32)(OFF + (s - 1) * L + k) : (u32)(s * k * TSTEP);
-                    y[b][s] = F::template twiddle<INV>(y[b][s], tw_at<typename F::W>(tw, e));
+                    y[b][s] = F::template twiddle<INV>(y[b][s], tw_at<W>(tw, e));
                 }
             }
             dft_r<F, r, INV>(y[b]);
