@@ -349,11 +349,15 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             {
                 const int tw0 = ll_opaque(t);
                 Layout L = layout(ll_opaque(p), tw0);
+                double wa[R], wb[R];
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const Pair q = pair_at(L);
-                    x[s] = cmk(x[s].re * wlds(q.e0, 0, 0), x[s].im * wlds(q.e1, 0, 1));
+                    wa[s] = wlds(q.e0, 0, 0);
+                    wb[s] = wlds(q.e1, 0, 1);
                 }
+#pragma unroll
+                for (int s = 0; s < R; ++s) x[s] = cmk(x[s].re * wa[s], x[s].im * wb[s]);
             }
             if constexpr (P2)
                 transform<F, M, G, R, K.R1, K.R2, K.R3, K.R4, false, true, PS, PSM>(x, lds, t, twp);
@@ -389,18 +393,27 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             {
                 const int tu = ll_opaque(t);
                 Layout L = layout(ll_opaque(p), tu);
+                double wa[R], wb[R];
+                int ba[R], bb[R];
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    const Pair q = pair_at(L);
+                    wa[s] = wlds(q.e0, 1, 0);
+                    wb[s] = wlds(q.e1, 1, 1);
+                    ba[s] = q.b0;
+                    bb[s] = q.b1;
+                }
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const u32 j = tu + G * s;
-                    const Pair q = pair_at(L);
-                    const double v0 = x[s].re * wlds(q.e0, 1, 0), v1 = x[s].im * wlds(q.e1, 1, 1);
+                    const double v0 = x[s].re * wa[s], v1 = x[s].im * wb[s];
                     double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
                     eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
                     if (j == 0 && has) r0 -= 2.0;
                     double cA, cB;
-                    const double x0 = split(r0, q.b0, cA);
+                    const double x0 = split(r0, ba[s], cA);
                     r1 += cA;
-                    const double x1 = split(r1, q.b1, cB);
+                    const double x1 = split(r1, bb[s], cB);
                     x[s] = cmk(x0, x1);
                     cout[s] = cB;
                 }
