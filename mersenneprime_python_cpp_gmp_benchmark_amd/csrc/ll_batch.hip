@@ -368,6 +368,9 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             // lanes, which the LDS serves without bank conflicts (a pad element per 16 made
             // every 16-byte load of it take two passes).
             const int tq = ll_opaque(t);
+            C64 wq[R];      // the pairing's table loads stay in flight across the exchange
+#pragma unroll
+            for (int q = 0; q < R; ++q) wq[q] = ld_table(tw, (u32)(tq + G * q));
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < R; ++q) lds[tq + G * q] = x[q];
@@ -378,7 +381,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const int km = k == 0 ? 0 : M - k;
                 const C64 zm = lds[km];
                 const C64 D = csub(x[q], cconj(zm));
-                const C64 w = ld_table(tw, (u32)k);
+                const C64 w = wq[q];
                 const C64 A = cmk(1.0 + w.re, w.im);
                 const C64 AD = cmul(A, csqr(D));
                 const C64 Z2 = csqr(x[q]);
