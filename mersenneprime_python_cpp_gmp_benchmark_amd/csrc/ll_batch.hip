@@ -234,44 +234,53 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
             g_wtab<WG>[i] = ld_table(wt, slot * llh::kWtabSlot + k);
         }
     }
-    // weight of exponent e of parity ODD (forward: slots 0, 1; inverse: slots 2, 3)
-    auto wlds = [&](u32 e, int inv, int odd) {
-        const u32 lo = (u32)odd * ((smask + 1) >> 1) + ((e >> 1) & (smask >> 1));
-        return g_wtab<WG>[(2 * inv) * kSlot<WG> + lo] * g_wtab<WG>[(2 * inv + 1) * kSlot<WG> + (e >> log_s)];
+    // Weight of the digit whose walk value is E = 4 e (e its weight exponent, of parity ODD;
+    // forward: slots 0, 1; inverse: slots 2, 3).  With the low slot stored as even entries,
+    // then odd ones, entry e mod S sits at byte 4 (e mod S) for even e and at byte
+    // 4 (S - 1) + 4 (e mod S) for odd e: both are E masked to 4 (S - 1), plus a constant.
+    auto wlds = [&](u32 E, int inv, int odd) {
+        const char *lo = reinterpret_cast<const char *>(&g_wtab<WG>[(2 * inv) * kSlot<WG>]);
+        const char *hi = reinterpret_cast<const char *>(&g_wtab<WG>[(2 * inv + 1) * kSlot<WG>]);
+        const u32 lob = (u32)odd * 4u * smask + (E & (4u * smask));
+        const u32 hib = (E >> (log_s + 2)) << 3;
+        return *reinterpret_cast<const double *>(lo + lob) * *reinterpret_cast<const double *>(hi + hib);
     };
     // Digits 2j, 2j + 1 of element j = t + G s, s = 0 .. R-1, in order.  With P = p mod N and
     // F = p div N (p > 64 is prime, so coprime to N, and 0 < P < N), the digit of weight
     // exponent e ((-d p) mod N for digit index d) has F + [e < P] bits, and the next digit's
     // exponent is (e - P) mod N; element j + G's low digit has (e - 2 G p) mod N.  This equals
-    // the residue walk of DigitWalk digit for digit.
+    // the residue walk of DigitWalk digit for digit.  The walk carries E = 4 e (and 4 P,
+    // 4 step, mod 4 N), which is the low table's byte offset up to a mask.
     struct Layout {
-        u32 P, F, step, e;
+        u32 P4, F, step4, E;
     };
-    auto sub_mod = [](u32 a, u32 b) -> u32 {      // (a - b) mod N for a, b < N
+    constexpr u32 N4 = 4u * N;
+    auto sub_mod = [](u32 a, u32 b) -> u32 {      // (a - b) mod 4N for a, b < 4N
         if constexpr (P2) {
-            return (a - b) & nmask;
+            return (a - b) & (N4 - 1);
         } else {
-            // for a < b the difference wraps to at least 2^32 - N and d + N is the residue;
-            // otherwise d < N <= d + N: the smaller of the two is (a - b) mod N
+            // for a < b the difference wraps to at least 2^32 - 4N and d + 4N is the residue;
+            // otherwise d < 4N <= d + 4N: the smaller of the two is (a - b) mod 4N
             const u32 d = a - b;
-            return min(d, d + N);
+            return min(d, d + N4);
         }
     };
     auto layout = [&](u32 pp, int tt) {
         Layout L;
-        L.P = pp % N;
+        const u32 P = pp % N;
         L.F = pp / N;
-        L.step = 2u * (u32)G * (L.P % (u32)R);     // (2 G P) mod N, as N = 2 G R
-        L.e = sub_mod(0u, (2u * (u32)tt * L.P) % N);
+        L.P4 = 4u * P;
+        L.step4 = 8u * (u32)G * (P % (u32)R);      // 4 ((2 G P) mod N), as N = 2 G R
+        L.E = sub_mod(0u, 4u * ((2u * (u32)tt * P) % N));
         return L;
     };
     auto pair_at = [&](Layout &L) {
         Pair q;
-        q.e0 = L.e;
-        q.e1 = sub_mod(q.e0, L.P);
-        q.b0 = (int)(L.F + (q.e0 < L.P ? 1u : 0u));
-        q.b1 = (int)(L.F + (q.e1 < L.P ? 1u : 0u));
-        L.e = sub_mod(q.e0, L.step);
+        q.e0 = L.E;
+        q.e1 = sub_mod(q.e0, L.P4);
+        q.b0 = (int)(L.F + (q.e0 < L.P4 ? 1u : 0u));
+        q.b1 = (int)(L.F + (q.e1 < L.P4 ? 1u : 0u));
+        L.E = sub_mod(q.e0, L.step4);
         return q;
     };
 
