@@ -387,7 +387,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int k = tq + G * q;
-                const int km = k == 0 ? 0 : M - k;
+                const int km = q == 0 && tq == 0 ? 0 : M - k;     // k = 0 only for q = 0, t = 0
                 const C64 zm = lds[km];
                 const C64 D = csub(x[q], cconj(zm));
                 const C64 w = wq[q];
@@ -421,7 +421,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                     const double v0 = x[s].re * wa[s], v1 = x[s].im * wb[s];
                     double r0 = __builtin_rint(v0), r1 = __builtin_rint(v1);
                     eit = fmax(eit, fmax(fabs(v0 - r0), fabs(v1 - r1)));
-                    if (j == 0 && has) r0 -= 2.0;
+                    if (s == 0 && tu == 0 && has) r0 -= 2.0;     // j = 0 only for s = 0, t = 0
                     double cA, cB;
                     const double x0 = split(r0, ba[s], cA);
                     r1 += cA;
@@ -449,7 +449,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const u32 j = tc + G * s;
-                    const double cin = ldsd[j == 0 ? M - 1 : j - 1];
+                    const double cin = ldsd[s == 0 && tc == 0 ? M - 1 : j - 1];
                     const Pair q = pair_at(L);
                     double cC, cD;
                     const double x0 = split(x[s].re + cin, q.b0, cC);
@@ -470,7 +470,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const u32 j = tc + G * s;
-                    x[s].re += ldsd[j == 0 ? M - 1 : j - 1];
+                    x[s].re += ldsd[s == 0 && tc == 0 ? M - 1 : j - 1];
                 }
             }
         } else {
@@ -496,7 +496,7 @@ __device__ __attribute__((noinline)) void run_class(const llh::ClassDesc &cd, co
                 const u32 e = (0u - j * pu) & nmask;
                 const int b = (int)((pu >> log_n) + (e < (pu & nmask) ? 1u : 0u));   // as pair_at
                 i64 v = gl_to_i64(gl_mul(x[s], ld_table(wi, e)));
-                if (j == 0 && has) v -= 2;
+                if (s == 0 && tu == 0 && has) v -= 2;
                 const i64 c = (v + ((i64)1 << (b - 1))) >> b;
                 x[s] = (u64)(v - (c << b));
                 cout[s] = c;
