@@ -638,10 +638,11 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
         // partner of (row, k_w): row 0 -> (0, -k_w); row H/2 -> (H/2, W-1-k_w);
         // otherwise -> (H - row, W-1-k_w), held by the other group of this workgroup
         const T *plds = (row == 0 || row == H / 2) ? lds : lds_all + (1 - g) * REG;
+        const int kwb = (row == 0 ? W : W - 1) - tt;
 #pragma unroll
         for (int q = 0; q < kR; ++q) {
-            const int kw = tt + GW * q;
-            const int kwp = row == 0 ? (kw == 0 ? 0 : W - kw) : (W - 1 - kw);
+            // k_w = tt + GW q is 0 only for q = 0, tt = 0 (row 0 pairs it with itself)
+            const int kwp = q == 0 && row == 0 && tt == 0 ? 0 : kwb - GW * q;
             const C64 zm = plds[kwp];
             const C64 D = csub(x[q], cconj(zm));
             const C64 AD = cmul(cmk(1.0 + wk[q].re, wk[q].im), csqr(D));
