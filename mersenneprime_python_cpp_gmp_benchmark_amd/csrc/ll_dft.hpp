@@ -53,10 +53,20 @@ template <class W, int NT, int R1, int R2, int R3, int R4, int R5>
 LL_DEV void stage_pass_table(W *dst, const void *src, int tid) {
     typedef PassTable<R1, R2, R3, R4, R5> PT;
     const W LL_GLOBAL *g = (const W LL_GLOBAL *)src;
+    constexpr int NI = (PT::SIZE + NT - 1) / NT;
+    // All of a thread's loads are issued before its first store, so their round trips overlap
+    // (a guarded load-then-store per entry waits out each load in turn).  Lanes past the end
+    // load the last entry and store nothing.
+    W v[NI];
 #pragma unroll
-    for (int i = 0; i < (PT::SIZE + NT - 1) / NT; ++i) {
+    for (int i = 0; i < NI; ++i) {
         const int e = tid + NT * i;
-        if (PT::SIZE % NT == 0 || e < PT::SIZE) dst[e] = ld_table(g, PT::exponent(e));
+        v[i] = ld_table(g, PT::exponent(PT::SIZE % NT == 0 || e < PT::SIZE ? e : PT::SIZE - 1));
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int e = tid + NT * i;
+        if (PT::SIZE % NT == 0 || e < PT::SIZE) dst[e] = v[i];
     }
 }
 
