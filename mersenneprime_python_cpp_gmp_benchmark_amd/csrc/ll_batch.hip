@@ -169,7 +169,8 @@ LL_DEV gptr<T> uniform_table(const void *p) {
     return (gptr<T>)(((u64)hi << 32) | lo);
 }
 
-// Bits and weight exponents of the two digits 2j, 2j + 1 of FP64 element j.
+// Bits and weight exponents (as 4 e, the walk's scaling) of the two digits 2j, 2j + 1 of FP64
+// element j.
 struct Pair {
     int b0, b1;
     u32 e0, e1;
