@@ -639,7 +639,9 @@ struct LargeJob {
     bool fault = false;
 
     // 0 = started; kStNotRun in r.status = no kernels for this length
-    int begin(u64 p_, u64 iters_, int transform_, u64 N_, hipStream_t st_) {
+    // alone: no other large job runs beside this one (the fused column pass then orders its
+    // blocks by XCD; with concurrent jobs the workgroups are not dealt to XCDs in block order)
+    int begin(u64 p_, u64 iters_, int transform_, u64 N_, hipStream_t st_, bool alone) {
         p = p_; iters = iters_; transform = transform_; N = N_; st = st_;
         done = 0;
         r = BatchResult();
@@ -690,6 +692,7 @@ struct LargeJob {
         A.ticket = (unsigned long long *)sync;
         A.flags = (unsigned *)((char *)sync + 8);
         A.fault = (unsigned *)((char *)errs + kGraphBlock * 8);
+        A.xcd_blocks = alone ? 1u : 0u;
         trace.clear();
         trace.reserve(iters);
         HIPCHK(hipEventCreate(&ev0));
@@ -864,7 +867,8 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
     auto start = [&](Slot &s, size_t i, u64 N) -> int {
         s.item = i;
         s.busy = true;
-        int r2 = s.job.begin(large[i].p, large[i].iters, transform, N, g_dev.fstream[&s - slots.data()]);
+        int r2 = s.job.begin(large[i].p, large[i].iters, transform, N, g_dev.fstream[&s - slots.data()],
+                             slots.size() == 1);
         if (r2) return r2;
         if (s.job.running()) return s.job.launch_block();
         return LLH_OK;

@@ -96,6 +96,7 @@ struct LargeArgs {
     unsigned long long *ticket;       // fused column pass: launch counter (only grows)
     unsigned *flags;                  // [W / 4] per-block carry-out sequence numbers
     unsigned *fault;                  // raised when a carry hand-off wait gives up
+    u32 xcd_blocks;                   // fused column pass: XCD-aware block order (job runs alone)
 };
 
 struct LargeLaunch {

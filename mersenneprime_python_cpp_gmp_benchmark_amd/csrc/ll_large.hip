@@ -532,7 +532,8 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
 // (b mod 8); mapping runs of NBLK / 8 consecutive blocks to one XCD keeps each block's
 // predecessor, whose carries and flag it waits for, on the same XCD for all but 8 hand-offs,
 // and neighbouring blocks, whose 64-byte row segments share HBM lines, on one L2.  Below 128
-// blocks (a quarter of the CUs or fewer in use) the plain order measured faster.
+// blocks (a quarter of the CUs or fewer in use) the plain order measured faster, and so it did
+// with several jobs' kernels running at once (LargeArgs::xcd_blocks is then 0).
 template <int NBLK>
 LL_DEV int xcd_block(int b) {
     if constexpr (NBLK % 8 == 0 && NBLK >= 128) return (b % 8) * (NBLK / 8) + b / 8;
@@ -552,7 +553,7 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fused(LargeArgs A) {
     __shared__ unsigned long long s_err;
     __shared__ u32 s_seq;
     const int tid = threadIdx.x, c = tid % kCW, t = tid / kCW;
-    const int blk = xcd_block<NBLK>(blockIdx.x), w = blk * kCW + c;
+    const int blk = A.xcd_blocks ? xcd_block<NBLK>(blockIdx.x) : (int)blockIdx.x, w = blk * kCW + c;
     if (tid == 0) {
         s_seq = (u32)(atomicAdd(A.ticket, 1ull) / (unsigned long long)NBLK) + 1u;
         s_err = 0;
