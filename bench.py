@@ -78,6 +78,24 @@ def load_range(name: str):
     return out
 
 
+def host_info() -> dict:
+    """The host's CPU model and the CPUs this process may run on (what `nproc` reports)."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+
+
+def default_cpu_threads() -> int:
+    """nproc, capped at 16: a GPU box gives each GPU a 16-CPU share, whatever nproc shows there."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 def fp64_flops_per_iter(n: int) -> float:
     return 5.0 * n * math.log2(n) + 6.5 * n
 
@@ -89,10 +107,43 @@ def cpu_baseline(ps: list[int], every: int, threads: int) -> dict:
     t0 = time.perf_counter()
     O.ll_many(sample, engine=0, threads=threads)
     dt = time.perf_counter() - t0
+    total = sum(p - 2 for p in ps if p > 2)
     return {"value": round(iters / dt, 1), "unit": "LL iters/s", "cores": threads, "kind": "port",
             "sample": f"GMP engine (restates lucaslehmer.cpp:29-56), full LL of every {every}th prime "
                       f"<= 100001 ({len(sample)} exponents, {iters} squarings, {dt:.1f} s wall), "
-                      f"one exponent per thread, GMP {O.gmp_version()}"}
+                      f"one exponent per thread, GMP {O.gmp_version()}; all {len(ps)} exponents "
+                      f"({total} squarings) extrapolate to {total / (iters / dt):.0f} s",
+            "host": host_info()}
+
+
+def c1_side_by_side(llhip, threads: int, log) -> dict:
+    """All 1229 primes p <= 10001, full tests: the two GPU transforms (one call each) beside the
+    oracle's two CPU engines (GMP, lucaslehmer.cpp:29-56; FP64 IBDWT, :256-329 on its own FFT)
+    on `threads` host threads, one exponent per thread (PerfNumMultiCLL.py:189-193).  Every
+    Res64 is checked against the committed GMP fixture."""
+    from oracle import oracle as O
+    fx = load_range("c1_gmp_p10001.csv") or {}
+    ps = sorted(fx) if fx else O.primes(2, 10001)
+    squarings = sum(p - 2 for p in ps if p > 2)
+    out = {"exponents": len(ps), "squarings": squarings, "host_threads": threads}
+    for name, tr in (("gpu_fp64", llhip.FP64), ("gpu_ntt", llhip.NTT)):
+        llhip.batch(ps, 0, tr)                       # tables and code objects warm
+        t0 = time.perf_counter()
+        r = llhip.batch(ps, 0, tr)
+        dt = time.perf_counter() - t0
+        diffs = sum(1 for p, v, x in zip(ps, r["verdict"], r["res64"]) if fx and fx[p] != (int(v != 0), int(x)))
+        out[name] = {"seconds": round(dt, 4), "iters_per_s": round(squarings / dt, 1), "res64_diffs": diffs}
+    for name, eng in (("cpu_gmp", 0), ("cpu_fp64", 1)):
+        t0 = time.perf_counter()
+        got = O.ll_many(ps, engine=eng, threads=threads)
+        dt = time.perf_counter() - t0
+        diffs = sum(1 for p, (v, x) in zip(ps, got) if fx and fx[p] != (int(v), int(x)))
+        out[name] = {"seconds": round(dt, 3), "iters_per_s": round(squarings / dt, 1), "res64_diffs": diffs}
+        log(f"c1 {name}: {dt:.2f} s on {threads} threads")
+    out["gpu_fp64_vs_cpu_gmp"] = round(out["cpu_gmp"]["seconds"] / out["gpu_fp64"]["seconds"], 1)
+    out["note"] = ("reference's own figure: 0.79 s on 16 threads of a Xeon w5-3435X, engine not stated "
+                   "(README.md:78-84); the CPU FP64 engine is the oracle's (no FFTW in this image)")
+    return out
 
 
 def cpu_fp64_engine(ps: list[int], every: int, threads: int, iters: int = 2000) -> dict:
@@ -160,8 +211,9 @@ def main() -> int:
     ap.add_argument("--workload", choices=["c3", "c3-ntt", "c2", "p24-fp64", "p24-ntt", "copy"], default="c3")
     ap.add_argument("--n", type=int, default=0, help="p24-fp64: transform length (0 = conservative rule)")
     ap.add_argument("--iters", type=int, default=0, help="squarings for the single-exponent workloads")
-    ap.add_argument("--cpu-every", type=int, default=200, help="CPU baseline sample: every k-th prime")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-every", type=int, default=48, help="CPU baseline sample: every k-th prime")
+    ap.add_argument("--cpu-threads", type=int, default=default_cpu_threads(),
+                    help="host threads of the CPU baselines (default: nproc, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     a = ap.parse_args()
@@ -330,6 +382,8 @@ def main() -> int:
             gpu_us[P24] = round(1e6 / extra["p16777213_5000_iters"]["plain_conservative"]["iters_per_s"], 2)
         extra["cpu_gmp_square"] = cpu_square([P20, P24], gpu_us)
         log(f"cpu gmp square: {extra['cpu_gmp_square']}")
+        if a.workload == "c3" and not a.no_extra:
+            extra["c1_all_primes_to_10001"] = c1_side_by_side(llhip, a.cpu_threads, log)
 
     value = a.steps * total_iters / wall
     line = {

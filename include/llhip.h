@@ -55,24 +55,33 @@ uint64_t llh_fft_length(uint64_t p, double max_bpw);
  * For LLH_NTT this extends ntt_length cuda/ll_ntt.cu:176-180 down to N = 16. */
 uint64_t llh_batch_length(uint64_t p, int transform);
 
-/* Smallest FP64 length the engines run (batched powers of two, then the large engine's
- * {1,3,5,7,15} * 2^k) with p/N <= max_bpw (<= 0: the bits-per-word knob).  0 = none. */
+/* FP64 length of the `lucaslehmer` module's FFT functions: the conservative rule's length
+ * (llh_fft_length, lucaslehmer.cpp:244-254) when an engine runs it, else the shortest length of
+ * the engines above it (max_bpw <= 0: the bits-per-word knob).  0 = none. */
 uint64_t llh_fp64_length(uint64_t p, double max_bpw);
 
 /* Reference NTT length rule (N in 64..8192, 0 = too large).  cuda/ll_ntt.cu:176-180. */
 uint64_t llh_ntt_length(uint64_t p);
 
-/* Largest exponent `transform` handles in the batched engine.  cuda/ll_ntt.cu:279. */
+/* LLH_NTT: the reference's bound, 8192 * max_bits_per_word(8192) = 204,800, the largest p its
+ * length rule covers (max_exponent cuda/ll_ntt.cu:279).  Every p above it runs too, on the large
+ * engine.  LLH_FP64 (no reference counterpart): the batched FP64 engine's bound, as
+ * llh_batch_max_exponent. */
 uint64_t llh_max_exponent(int transform);
+
+/* Largest exponent `transform` handles in the batched (LDS-resident) engine: NTT 4096 * 25 =
+ * 102,400; FP64 the batch bits-per-digit cap times 8192.  Larger p run on the large engine.
+ * No reference counterpart (the reference's single kernel has one bound, cuda/ll_ntt.cu:279). */
+uint64_t llh_batch_max_exponent(int transform);
 
 /* Bits-per-word cap of the FP64 length rule (default 18).
  * Replaces set_fft_bits_per_word lucaslehmer.cpp:368-369 (atomic here, not a plain static). */
 int llh_set_fft_bits_per_word(double bits);
 double llh_get_fft_bits_per_word(void);
 
-/* Bits-per-digit cap of the batched FP64 engine's power-of-two lengths (default 20: its worst
- * round-off over every prime <= 100001 is 0.053).  No reference counterpart; the reference uses
- * one cap for every length. */
+/* Bits-per-digit cap of the batched FP64 engine's lengths (default 21: its worst round-off over
+ * every prime <= 100001 is then 0.21, half the 0.4 rollback threshold).  No reference
+ * counterpart; the reference uses one cap for every length. */
 int llh_set_batch_bits_per_word(double bits);
 double llh_get_batch_bits_per_word(void);
 
@@ -109,6 +118,27 @@ int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, i
  * (HIP events on the library's stream) and how many launches it took (1 + rollback launches).
  * No reference counterpart: measurement hook for the roofline report. */
 int llh_last_launch_stats(double *kernel_ms, uint32_t *launches);
+
+/* Large-engine squarings queued by the last llh_batch / llh_lucas_lehmer call (those of a run
+ * that rolled back included), how many rollbacks it took (both engines), and the iteration the
+ * last rollback resumed from (0: from s_0).  No reference counterpart: the reference aborts at
+ * round-off 0.35 and redoes the whole test with GMP (lucaslehmer.cpp:296, :331-335). */
+int llh_last_run_stats(uint64_t *large_squarings, uint32_t *rollbacks, uint64_t *resumed_at);
+
+/* Checkpoint interval of the large FP64 engine in blocks of 200 squarings (default 1; 0 = a
+ * rollback restarts from s_0).  After each such block the state is copied on the device; when a
+ * later block's round-off exceeds 0.4, the last checkpoint whose blocks all stayed within 0.4 is
+ * normalised on the device, re-split exactly at the next length, and the test resumes from its
+ * iteration.  No reference counterpart (lucaslehmer.cpp:296 aborts the whole FP64 test). */
+int llh_set_checkpoint_blocks(int blocks);
+int llh_get_checkpoint_blocks(void);
+
+/* Host run of the per-digit arithmetic of the device re-layout (the checkpoint rollback's
+ * N -> N' step): `from` holds the n_from canonical digits of X in [0, 2^p) (digit d has the
+ * c_{d+1} - c_d bits of X from c_d = ceil(d p / n_from)); `to` receives n_to near-balanced digits
+ * (|digit| <= 2^(b-1)) of X mod 2^p - 1.  No device call.  No reference counterpart: it exists so
+ * the re-layout can be checked on the CPU against the oracle (orc_limbs_to_digits). */
+int llh_relayout_host(uint64_t p, uint64_t n_from, const int64_t *from, uint64_t n_to, int64_t *to);
 
 /* One exponent at an explicit transform length n_force (one of llh_supported_lengths; 0 = the
  * engine's length rule).  flags bit 0: no rollback (an FP64 round-off above 0.4 ends the run with

@@ -3,7 +3,7 @@
 Native parts (built in-tree by ``build.py``):
   libllhip.so        HIP kernels for gfx950 + the C ABI of include/llhip.h
   lucaslehmer*.so    pybind11 module with the reference's API (import name ``lucaslehmer``)
-  lucaslehmer_hip*.so  batch module, counterpart of the reference's ``lucaslehmer_cuda``
+  lucaslehmer_cuda*.so batch module with the reference's GPU module name and API
 
 ``add_dropin_path()`` puts this directory on ``sys.path`` so that ``import lucaslehmer``
 resolves to the drop-in, exactly as the reference's module is imported.

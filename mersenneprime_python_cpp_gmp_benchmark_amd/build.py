@@ -2,7 +2,7 @@
 
   libllhip.so                          HIP kernels + the C ABI of include/llhip.h (hipcc)
   lucaslehmer<EXT_SUFFIX>              pybind11 drop-in for the reference `lucaslehmer`
-  lucaslehmer_hip<EXT_SUFFIX>          pybind11 counterpart of the reference `lucaslehmer_cuda`
+  lucaslehmer_cuda<EXT_SUFFIX>         pybind11 batch module under the reference's GPU module name
 
 Everything lands next to this file so it travels with the source tree.  Run
 ``python -m mersenneprime_python_cpp_gmp_benchmark_amd.build`` or call ``build()``.
@@ -68,7 +68,7 @@ def build_modules() -> None:
     common = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden", *inc]
     link = [f"-L{HERE}", "-lllhip", "-Wl,-rpath,$ORIGIN"]
     _run([*common, os.path.join(CSRC, "lucaslehmer_module.cpp"), "-o", _ext("lucaslehmer"), *link])
-    _run([*common, os.path.join(CSRC, "lucaslehmer_hip_module.cpp"), "-o", _ext("lucaslehmer_hip"), *link])
+    _run([*common, os.path.join(CSRC, "lucaslehmer_cuda_module.cpp"), "-o", _ext("lucaslehmer_cuda"), *link])
 
 
 def build() -> None:

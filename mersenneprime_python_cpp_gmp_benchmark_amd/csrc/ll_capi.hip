@@ -10,6 +10,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -39,6 +40,8 @@ int large_blocks(const LargeLaunch &L);
 hipError_t large_squarings(const LargeLaunch &L, LargeArgs A, int k, hipStream_t st);
 hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
 hipError_t large_normalize(int transform, const void *elem, i64 *dig, const LargeArgs &A, void *out, hipStream_t st);
+hipError_t large_relayout(int transform, const i64 *dig, u32 p, u64 n, u64 n2, void *elem2, hipStream_t st);
+void relayout_host(const i64 *dig, u64 p, u64 n, u64 n2, i64 *out);
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -74,6 +77,9 @@ static std::map<std::pair<int, u64>, LargeTables> g_large_tables;
 static std::vector<double> g_err_trace;     // per-iteration max round-off of the last large run
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
+static u64 g_last_squarings = 0;          // squarings the large engine ran in the last call (kept or not)
+static u32 g_last_rollbacks = 0;          // rollbacks in the last call
+static u64 g_last_resumed_at = 0;         // iteration the last rollback resumed from (0: from s_0)
 static std::atomic<double> g_bpw{18.0};        // reference knob: the conservative length rule
 // Bits-per-digit cap of the batched FP64 engine (lengths up to 8192).  The worst round-off over
 // every prime <= 100001 roughly doubles per half bit: 0.05 at 19.75-20, 0.10 at 20.25-20.5,
@@ -615,35 +621,66 @@ struct NormOutH {
     int status;
 };
 
-// One exponent on the large engine at length N, as a resumable job.  Round-off is checked after
-// every block of kGraphBlock squarings; above 0.4 the job stops with kStRoundoff.  begin() allocates and
-// starts, each launch_block() queues up to kGraphBlock squarings (a captured hipGraph) on the
-// job's stream, wait_block() reads their round-off back, finish() normalises.  Several jobs run
-// at once on the library's streams: one exponent of this size leaves most of the GPU idle.
+// One exponent on the large engine at length N, as a resumable job.  Squarings are queued in
+// blocks of kGraphBlock (a captured hipGraph each), up to two blocks in flight, so the device
+// never waits for the host's round-off check of the block before.  After each block the state
+// (the element array and the pending row-segment carries) is copied into one of three
+// checkpoint slots; a block whose round-off stays within 0.4 makes its checkpoint the last good
+// one.  Above 0.4 the job stops with kStRoundoff, and rollback_state() turns the last good
+// checkpoint into canonical digits (carries applied, normalised on the device) from which the
+// next length resumes at that checkpoint's iteration.  Several jobs run at once on the
+// library's streams: one exponent of this size leaves most of the GPU idle.
+struct Resume {
+    void *dig = nullptr;     // canonical digits of the checkpoint at length n (device, n i64)
+    size_t bytes = 0;
+    u64 n = 0, done = 0;     // its length and the squarings it has done
+};
+
+static std::atomic<int> g_ckpt_blocks{1};   // checkpoint every k graph blocks; 0: restart from s_0
+
 struct LargeJob {
-    u64 p = 0, iters = 0, N = 0, done = 0, k = 0;
+    struct Block {
+        u64 k;          // squarings in the block
+        int e;          // round-off slot set (0 or 1)
+        int ck;         // checkpoint slot written after it (-1: none)
+        u64 done_after;
+    };
+    u64 p = 0, iters = 0, N = 0, done = 0, queued = 0;
+    u64 nblocks = 0;                  // blocks queued so far
     int transform = kFP64;
     hipStream_t st = nullptr;
     LargeLaunch L;
     LargeArgs A;
-    void *elem = nullptr, *carry = nullptr, *dig = nullptr, *errs = nullptr, *nout = nullptr, *sync = nullptr;
-    double *eh = nullptr;             // pinned: round-off slots + fault word of the last block
-    NormOutH *noh = nullptr;          // pinned
-    size_t errs_bytes = 0;
+    void *elem = nullptr, *carry = nullptr, *dig = nullptr, *nout = nullptr, *sync = nullptr;
+    void *errs[2] = {nullptr, nullptr};      // per-squaring round-off slots of a block in flight
+    double *eh[2] = {nullptr, nullptr};      // pinned: those slots + the fault word
+    hipEvent_t evb[2] = {nullptr, nullptr};  // recorded after each in-flight block's read-back
+    void *ck_elem[3] = {}, *ck_carry[3] = {};
+    u64 ck_done[3] = {0, 0, 0};
+    int ck_next = 0, ck_good = -1;
+    NormOutH *noh = nullptr;                 // pinned
+    size_t errs_bytes = 0, eh_bytes = 0;
     size_t bytes_elem = 0, bytes_carry = 0, bytes_dig = 0, bytes_sync = 0;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::deque<Block> fl;                    // blocks in flight, oldest first
     std::vector<double> trace;
     BatchResult r = BatchResult();
     bool fault = false;
+    u64 squarings = 0;                       // squarings queued on the device (kept or not)
 
-    // 0 = started; kStNotRun in r.status = no kernels for this length
-    // alone: no other large job runs beside this one (the fused column pass then orders its
-    // blocks by XCD; with concurrent jobs the workgroups are not dealt to XCDs in block order)
-    int begin(u64 p_, u64 iters_, int transform_, u64 N_, hipStream_t st_, bool alone) {
+    // 0 = started; kStNotRun in r.status = no kernels for this length.  alone: no other large
+    // job runs beside this one (the fused column pass then orders its blocks by XCD; with
+    // concurrent jobs the workgroups are not dealt to XCDs in block order).  from: resume from
+    // these canonical digits (a checkpoint at another length) instead of s_0 = 4.
+    int begin(u64 p_, u64 iters_, int transform_, u64 N_, hipStream_t st_, bool alone, const Resume *from) {
         p = p_; iters = iters_; transform = transform_; N = N_; st = st_;
-        done = 0;
+        done = queued = nblocks = 0;
+        ck_next = 0; ck_good = -1;
+        fl.clear();
+        fault = false;
+        squarings = 0;
         r = BatchResult();
         r.status = kStOk;
         const u64 M = transform == kFP64 ? N / 2 : N;
@@ -658,31 +695,40 @@ struct LargeJob {
         const int log_n = (N & (N - 1)) ? 0 : ceil_log2(N);
         const size_t esz = transform == kFP64 ? 16 : 8;
         const u64 nblk = (u64)large_blocks(L);
-        // errs: kGraphBlock round-off slots, then the hand-off fault word
-        errs_bytes = (kGraphBlock + 1) * 8;
-        // sync: the fused pass's launch counter, then one flag per column block
-        const size_t sync_bytes = 8 + nblk * 4;
+        errs_bytes = kGraphBlock * 8;
+        eh_bytes = (kGraphBlock + 1) * 8;
+        // sync: the fused pass's launch counter, the hand-off fault word, one flag per column block
+        bytes_sync = 16 + nblk * 4;
         bytes_elem = M * esz;
         bytes_carry = nblk * H * 8;
         bytes_dig = N * 8;
-        bytes_sync = sync_bytes;
         if ((rc = cached_malloc(&elem, bytes_elem)) || (rc = cached_malloc(&carry, bytes_carry)) ||
-            (rc = cached_malloc(&dig, bytes_dig)) || (rc = cached_malloc(&errs, errs_bytes)) ||
-            (rc = cached_malloc(&nout, sizeof(NormOutH))) || (rc = cached_malloc(&sync, sync_bytes)) ||
-            (rc = cached_host_malloc((void **)&eh, errs_bytes)) ||
-            (rc = cached_host_malloc((void **)&noh, sizeof(NormOutH))))
+            (rc = cached_malloc(&dig, bytes_dig)) || (rc = cached_malloc(&nout, sizeof(NormOutH))) ||
+            (rc = cached_malloc(&sync, bytes_sync)) || (rc = cached_host_malloc((void **)&noh, sizeof(NormOutH))))
             return rc;
-        HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
-        HIPCHK(hipMemsetAsync(carry, 0, nblk * H * 8, st));
-        HIPCHK(hipMemsetAsync(sync, 0, sync_bytes, st));
-        if (transform == kFP64) {
-            const double four = 4.0;
-            HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+        for (int e = 0; e < 2; ++e)
+            if ((rc = cached_malloc(&errs[e], errs_bytes)) || (rc = cached_host_malloc((void **)&eh[e], eh_bytes)))
+                return rc;
+        if (transform == kFP64 && g_ckpt_blocks.load() > 0)
+            for (int c = 0; c < 3; ++c)
+                if ((rc = cached_malloc(&ck_elem[c], bytes_elem)) || (rc = cached_malloc(&ck_carry[c], bytes_carry)))
+                    return rc;
+        HIPCHK(hipMemsetAsync(carry, 0, bytes_carry, st));
+        HIPCHK(hipMemsetAsync(sync, 0, bytes_sync, st));
+        if (from) {
+            HIPCHK(large_relayout(transform, (const i64 *)from->dig, (u32)p, from->n, N, elem, st));
+            done = queued = from->done;
         } else {
-            const u64 four = 4;
-            HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
+            if (transform == kFP64) {
+                const double four = 4.0;
+                HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+            } else {
+                const u64 four = 4;
+                HIPCHK(hipMemcpyAsync(elem, &four, 8, hipMemcpyHostToDevice, st));
+            }
         }
-        A.elem = elem; A.carry = carry; A.err = (unsigned long long *)errs;
+        A.elem = elem; A.carry = carry; A.err = (unsigned long long *)errs[0];
         A.twW = T.twW; A.twWi = T.twWi; A.twH = T.twH; A.twHi = T.twHi;
         A.tmL = T.tmL; A.tmH = T.tmH; A.tmLi = T.tmLi; A.tmHi = T.tmHi;
         A.wfL = T.wfL; A.wfH = T.wfH; A.wiL = T.wiL; A.wiH = T.wiH;
@@ -690,61 +736,123 @@ struct LargeJob {
         A.n = N; A.inv_n = 1.0 / (double)N;
         A.p_mod_n = (u32)(p % N); A.p_div_n = (u32)(p / N);
         A.ticket = (unsigned long long *)sync;
-        A.flags = (unsigned *)((char *)sync + 8);
-        A.fault = (unsigned *)((char *)errs + kGraphBlock * 8);
+        A.fault = (unsigned *)((char *)sync + 8);
+        A.flags = (unsigned *)((char *)sync + 16);
         A.xcd_blocks = alone ? 1u : 0u;
         trace.clear();
-        trace.reserve(iters);
+        trace.reserve(iters - done);
+        for (int e = 0; e < 2; ++e) HIPCHK(hipEventCreateWithFlags(&evb[e], hipEventDisableTiming));
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
         HIPCHK(hipEventRecord(ev0, st));
+        if (from) HIPCHK(hipStreamSynchronize(st));   // the caller may reuse from->dig afterwards
         return LLH_OK;
     }
 
     bool running() const { return r.status == kStOk && done < iters && !fault; }
 
+    // queues the next block: its squarings, the read-back of their round-off, a checkpoint copy
     int launch_block() {
-        k = std::min<u64>(kGraphBlock, iters - done);
-        HIPCHK(hipMemsetAsync(errs, 0, errs_bytes, st));
+        const u64 k = std::min<u64>(kGraphBlock, iters - queued);
+        const int e = (int)(nblocks % 2);
+        LargeArgs Ab = A;
+        Ab.err = (unsigned long long *)errs[e];
+        HIPCHK(hipMemsetAsync(errs[e], 0, errs_bytes, st));
         if (k == (u64)kGraphBlock) {
-            if (!gexec) {
+            if (!gexec[e]) {
                 HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                hipError_t e = large_squarings(L, A, (int)k, st);
-                if (e != hipSuccess) {
+                hipError_t er = large_squarings(L, Ab, (int)k, st);
+                if (er != hipSuccess) {
                     hipGraph_t g2;
                     (void)hipStreamEndCapture(st, &g2);
-                    set_error(std::string("HIP: ") + hipGetErrorString(e) + " while capturing");
+                    set_error(std::string("HIP: ") + hipGetErrorString(er) + " while capturing");
                     return LLH_ERR_HIP;
                 }
-                HIPCHK(hipStreamEndCapture(st, &graph));
-                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                HIPCHK(hipStreamEndCapture(st, &graph[e]));
+                HIPCHK(hipGraphInstantiate(&gexec[e], graph[e], nullptr, nullptr, 0));
             }
-            HIPCHK(hipGraphLaunch(gexec, st));
+            HIPCHK(hipGraphLaunch(gexec[e], st));
         } else {
-            HIPCHK(large_squarings(L, A, (int)k, st));
+            HIPCHK(large_squarings(L, Ab, (int)k, st));
         }
-        HIPCHK(hipMemcpyAsync(eh, errs, errs_bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(eh[e], errs[e], errs_bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&eh[e][kGraphBlock], A.fault, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(evb[e], st));
+        queued += k;
+        squarings += k;
+        ++nblocks;
+        int ck = -1;
+        const int every = g_ckpt_blocks.load();
+        if (ck_elem[0] && every > 0 && (nblocks % (u64)every == 0) && queued < iters) {
+            ck = ck_next;
+            ck_next = (ck_next + 1) % 3;
+            HIPCHK(hipMemcpyAsync(ck_elem[ck], elem, bytes_elem, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(ck_carry[ck], carry, bytes_carry, hipMemcpyDeviceToDevice, st));
+            ck_done[ck] = queued;
+        }
+        fl.push_back(Block{k, e, ck, queued});
         return LLH_OK;
     }
 
-    // after the queued block: round-off bookkeeping and the rollback / fault decisions
-    int wait_block() {
-        HIPCHK(hipStreamSynchronize(st));
+    // queues blocks until two are in flight (or every squaring is queued)
+    int top_up() {
+        while (fl.size() < 2 && queued < iters) {
+            int rc = launch_block();
+            if (rc) return rc;
+        }
+        return LLH_OK;
+    }
+
+    // waits for the oldest block in flight; round-off bookkeeping and the rollback / fault decisions
+    int wait_oldest() {
+        if (fl.empty()) return LLH_OK;
+        const Block b = fl.front();
+        fl.pop_front();
+        HIPCHK(hipEventSynchronize(evb[b.e]));
         unsigned fw;
-        std::memcpy(&fw, &eh[kGraphBlock], sizeof fw);
+        std::memcpy(&fw, &eh[b.e][kGraphBlock], sizeof fw);
         if (fw) {
             fault = true;
             r.status = kStNotRun;
             return LLH_OK;
         }
         double bmax = 0;
-        for (u64 i = 0; i < k; ++i) {
-            trace.push_back(eh[i]);
-            bmax = std::max(bmax, eh[i]);
+        for (u64 i = 0; i < b.k; ++i) {
+            trace.push_back(eh[b.e][i]);
+            bmax = std::max(bmax, eh[b.e][i]);
         }
         r.maxerr = std::max(r.maxerr, bmax);
-        done += k;
-        if (transform == kFP64 && bmax > 0.4) r.status = kStRoundoff;
+        if (transform == kFP64 && bmax > 0.4) {
+            r.status = kStRoundoff;
+            return LLH_OK;
+        }
+        done = b.done_after;
+        if (b.ck >= 0) ck_good = b.ck;
+        return LLH_OK;
+    }
+
+    // After kStRoundoff: the canonical digits of the last good checkpoint, normalised on the
+    // device (false: no checkpoint, or its normalisation did not settle; restart from s_0).  On
+    // success the job's digit buffer moves to `out`.
+    int rollback_state(Resume &out, bool &ok) {
+        ok = false;
+        HIPCHK(hipStreamSynchronize(st));            // the blocks still in flight end
+        if (ck_good < 0 || fault) return LLH_OK;
+        LargeArgs Ac = A;
+        Ac.elem = ck_elem[ck_good];
+        Ac.carry = ck_carry[ck_good];
+        noh->status = 1;
+        HIPCHK(large_apply_carry(L, Ac, st));
+        HIPCHK(large_normalize(transform, Ac.elem, (i64 *)dig, Ac, nout, st));
+        HIPCHK(hipMemcpyAsync(noh, nout, sizeof(NormOutH), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (noh->status) return LLH_OK;
+        out.dig = dig;
+        out.bytes = bytes_dig;
+        out.n = N;
+        out.done = ck_done[ck_good];
+        dig = nullptr;
+        ok = true;
         return LLH_OK;
     }
 
@@ -780,21 +888,31 @@ struct LargeJob {
     }
 
     void release() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (int e = 0; e < 2; ++e) {
+            if (gexec[e]) (void)hipGraphExecDestroy(gexec[e]);
+            if (graph[e]) (void)hipGraphDestroy(graph[e]);
+            if (evb[e]) (void)hipEventDestroy(evb[e]);
+            g_bufs.put(errs[e], errs_bytes, false);
+            g_bufs.put(eh[e], eh_bytes, true);
+            gexec[e] = nullptr; graph[e] = nullptr; evb[e] = nullptr; errs[e] = nullptr; eh[e] = nullptr;
+        }
+        for (int c = 0; c < 3; ++c) {
+            g_bufs.put(ck_elem[c], bytes_elem, false);
+            g_bufs.put(ck_carry[c], bytes_carry, false);
+            ck_elem[c] = ck_carry[c] = nullptr;
+        }
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         g_bufs.put(elem, bytes_elem, false);
         g_bufs.put(carry, bytes_carry, false);
         g_bufs.put(dig, bytes_dig, false);
-        g_bufs.put(errs, errs_bytes, false);
         g_bufs.put(nout, sizeof(NormOutH), false);
         g_bufs.put(sync, bytes_sync, false);
-        g_bufs.put(eh, errs_bytes, true);
         g_bufs.put(noh, sizeof(NormOutH), true);
-        gexec = nullptr; graph = nullptr; ev0 = ev1 = nullptr;
-        elem = carry = dig = errs = nout = sync = nullptr;
-        eh = nullptr; noh = nullptr;
+        ev0 = ev1 = nullptr;
+        elem = carry = dig = nout = sync = nullptr;
+        noh = nullptr;
+        fl.clear();
     }
     ~LargeJob() { release(); }
 };
@@ -815,13 +933,17 @@ static std::vector<u64> large_lengths(int transform) {
 }
 
 // Default large-engine length for p.  FP64: the reference's conservative rule
-// (lucaslehmer.cpp:244-254: smallest {1,3,5} * 2^k with p/N <= bits-per-word cap);
+// (lucaslehmer.cpp:244-254: smallest {1,3,5} * 2^k with p/N <= bits-per-word cap), raised to the
+// smallest large-engine length at or above it (the large engine starts at N = 2^16, while the
+// rule gives 12288 .. 49152 for 172,032 < p <= 884,736, beyond the batched lengths' cap);
 // GL: the smallest power of two within the exactness bound (cuda/ll_ntt.cu:172-175).
 static u64 large_length(u64 p, int transform) {
     if (transform == kFP64) {
         const u64 N = fft_length_conservative(p, g_bpw.load());
-        LargeLaunch L;
-        return large_launch_for(transform, N / 2, L) ? N : 0;
+        if (!N) return 0;
+        for (u64 n : large_lengths(transform))
+            if (n >= N) return n;
+        return 0;
     }
     for (u64 N : large_lengths(transform))
         if ((p + N - 1) / N <= (u64)ntt_max_bits(N)) return N;
@@ -850,6 +972,19 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
         bool busy = false;
     };
     std::vector<Slot> slots(std::min<size_t>(kLargeJobs, large.size()));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // Every way out (an error included) first waits for the slots' streams, so no buffer goes
+    // back to the cache, and no graph is destroyed, while kernels queued on it may still run;
+    // then the events go.  (Declared after `slots`, so it runs before the slots are destroyed.)
+    struct Drain {
+        size_t n;
+        hipEvent_t &a, &b;
+        ~Drain() {
+            for (size_t i = 0; i < n; ++i) (void)hipStreamSynchronize(g_dev.fstream[i]);
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } drain{slots.size(), e0, e1};
     int rc;
     // the last exponent to finish leaves its round-off trace (a single exponent: its own)
     auto record = [&](Slot &s) {
@@ -863,14 +998,14 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
         if (n_used) n_used[w.idx] = (uint32_t)s.job.N;
         g_err_trace = s.job.trace;
     };
-    // start job for item i in slot s at length N (queues its first block)
-    auto start = [&](Slot &s, size_t i, u64 N) -> int {
+    // start job for item i in slot s at length N (queues its first blocks), from s_0 or a checkpoint
+    auto start = [&](Slot &s, size_t i, u64 N, const Resume *from) -> int {
         s.item = i;
         s.busy = true;
         int r2 = s.job.begin(large[i].p, large[i].iters, transform, N, g_dev.fstream[&s - slots.data()],
-                             slots.size() == 1);
+                             slots.size() == 1, from);
         if (r2) return r2;
-        if (s.job.running()) return s.job.launch_block();
+        if (s.job.running()) return s.job.top_up();
         return LLH_OK;
     };
     auto fill = [&](Slot &s) -> int {
@@ -880,12 +1015,11 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
                 status[large[i].idx] = kStNotRun;
                 continue;
             }
-            return start(s, i, large[i].n);
+            return start(s, i, large[i].n, nullptr);
         }
         s.busy = false;
         return LLH_OK;
     };
-    hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, g_dev.stream));
@@ -900,24 +1034,34 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
             any = true;
             LargeJob &j = s.job;
             if (j.running()) {
-                if ((rc = j.wait_block())) return rc;
+                if ((rc = j.wait_oldest())) return rc;
                 if (j.running()) {
-                    if ((rc = j.launch_block())) return rc;
+                    if ((rc = j.top_up())) return rc;
                     continue;
                 }
             }
             // the job has stopped: finished, rolled back or not run
-            if ((rc = j.finish())) return rc;
+            g_last_squarings += j.squarings;
+            j.squarings = 0;
             if (j.r.status == kStRoundoff && transform == kFP64 && !no_rollback) {
-                const u64 nx = next_large_length(j.N, transform);   // rollback: redo longer
+                const u64 nx = next_large_length(j.N, transform);   // rollback: resume longer
                 if (nx) {
+                    Resume from;
+                    bool ok = false;
+                    if ((rc = j.rollback_state(from, ok))) return rc;
+                    if ((rc = j.finish())) return rc;
                     const Pending &w = large[s.item];
                     secs[w.idx] += (double)(j.r.t1 - j.r.t0) / g_dev.wclk_hz;
                     errs[w.idx] = std::max(errs[w.idx], j.r.maxerr);
-                    if ((rc = start(s, s.item, nx))) return rc;
+                    g_last_rollbacks += 1;
+                    g_last_resumed_at = ok ? from.done : 0;
+                    rc = start(s, s.item, nx, ok ? &from : nullptr);
+                    g_bufs.put(from.dig, from.bytes, false);
+                    if (rc) return rc;
                     continue;
                 }
             }
+            if ((rc = j.finish())) return rc;
             record(s);
             if ((rc = fill(s))) return rc;
         }
@@ -931,8 +1075,6 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (!slots.empty()) {
         g_last_kernel_ms += ms;
         g_last_launches += 1;
@@ -956,6 +1098,9 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
     if (rc) return rc;
     g_last_kernel_ms = 0.0;
     g_last_launches = 0;
+    g_last_squarings = 0;
+    g_last_rollbacks = 0;
+    g_last_resumed_at = 0;
     std::vector<Pending> work, large;
     std::vector<double> secs(n, 0.0), errs(n, 0.0);
     for (size_t i = 0; i < n; ++i) {
@@ -1010,6 +1155,7 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
                 const u64 nx = next_batch_length(w.p, w.n, transform);   // rollback: redo longer
                 if (nx) {
                     again.push_back(Pending{w.p, w.iters, w.idx, batch_class_of(transform, nx), nx});
+                    g_last_rollbacks += 1;
                     continue;
                 }
             }
@@ -1074,16 +1220,24 @@ uint64_t llh_ntt_length(uint64_t p) { return ntt_length_ref(p); }
 uint64_t llh_fp64_length(uint64_t p, double max_bpw) {
     if (p < 64) return 0;
     if (!(max_bpw > 0)) max_bpw = g_bpw.load();
-    for (const auto &e : batch_ladder(kFP64)) {
-        const u64 N = e.first;
-        if ((double)p / (double)N <= max_bpw && batch_fits(p, N)) return N;
-    }
+    // the reference's length (lucaslehmer.cpp:244-254) when an engine runs it, else the shortest
+    // engine length above it (every such length keeps p / N <= max_bpw)
+    const u64 c = fft_length_conservative(p, max_bpw);
+    if (!c) return 0;
+    for (const auto &e : batch_ladder(kFP64))
+        if (e.first >= c && batch_fits(p, e.first)) return e.first;
     for (u64 N : large_lengths(kFP64))
-        if ((double)p / (double)N <= max_bpw) return N;
+        if (N >= c) return N;
     return 0;
 }
 
 uint64_t llh_max_exponent(int transform) {
+    // NTT: the reference's bound, 8192 * max_bits_per_word(8192) (cuda/ll_ntt.cu:279)
+    if (transform == kNTT) return (uint64_t)8192 * (uint64_t)ntt_max_bits(8192);
+    return (uint64_t)std::floor(g_batch_bpw.load() * (double)((uint64_t)1 << kFp64MaxLog));
+}
+
+uint64_t llh_batch_max_exponent(int transform) {
     if (transform == kNTT) return ((uint64_t)1 << kNttMaxLog) * (uint64_t)ntt_max_bits((uint64_t)1 << kNttMaxLog);
     return (uint64_t)std::floor(g_batch_bpw.load() * (double)((uint64_t)1 << kFp64MaxLog));
 }
@@ -1304,6 +1458,35 @@ int llh_mlp_predict(const float *params, const float *X, size_t n, float *y_out)
     HIPCHK(hipMemcpyAsync(y_out, B.Y, n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     mlp_free(B);
+    return LLH_OK;
+}
+
+int llh_last_run_stats(uint64_t *large_squarings, uint32_t *rollbacks, uint64_t *resumed_at) {
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    if (large_squarings) *large_squarings = g_last_squarings;
+    if (rollbacks) *rollbacks = g_last_rollbacks;
+    if (resumed_at) *resumed_at = g_last_resumed_at;
+    return LLH_OK;
+}
+
+int llh_set_checkpoint_blocks(int blocks) {
+    if (blocks < 0) {
+        set_error("checkpoint interval must be >= 0 blocks");
+        return LLH_ERR_ARG;
+    }
+    g_ckpt_blocks.store(blocks);
+    return LLH_OK;
+}
+
+int llh_get_checkpoint_blocks(void) { return g_ckpt_blocks.load(); }
+
+int llh_relayout_host(uint64_t p, uint64_t n_from, const int64_t *from, uint64_t n_to, int64_t *to) {
+    if (!from || !to || p < 3 || n_from < 1 || n_to < 1 || n_from > p || n_to > p || n_from > ((u64)1 << 24) ||
+        n_to > ((u64)1 << 24) || p >= ((u64)1 << 31)) {
+        set_error("llh_relayout_host: bad arguments");
+        return LLH_ERR_ARG;
+    }
+    relayout_host(from, p, n_from, n_to, to);
     return LLH_OK;
 }
 

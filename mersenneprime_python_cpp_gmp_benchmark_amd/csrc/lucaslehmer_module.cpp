@@ -40,9 +40,10 @@ void check(int rc) {
     if (rc != LLH_OK) throw std::runtime_error(std::string("llhip: ") + llh_last_error());
 }
 
-// One exponent.  The FP64 engine runs at the smallest length within the module's
-// bits-per-word knob (set_fft_bits_per_word, default 18, lucaslehmer.cpp:246); it still rolls
-// back to a longer length if the round-off check fires.
+// One exponent.  The FP64 engine runs at the reference's conservative length for the module's
+// bits-per-word knob (set_fft_bits_per_word, default 18, lucaslehmer.cpp:244-254), or, where no
+// engine has that length, the shortest engine length above it; it still rolls back to a longer
+// length if the round-off check fires.
 Run run_one(unsigned long p, unsigned long iters, int transform) {
     Run r;
     int rc;
@@ -83,9 +84,10 @@ bool trivial(unsigned long p, bool &v) {          // lucaslehmer.cpp:30-32
     return false;
 }
 
+// The argument rules set Res64 to 0 (the reference leaves the thread's previous value there).
 bool lucas_lehmer(unsigned long p) {
     bool v;
-    if (trivial(p, v)) return v;                  // Res64 untouched, as in the reference
+    if (trivial(p, v)) { g_last_res64 = 0; return v; }
     Run r = run_exact(p, 0);
     g_last_res64 = r.res64;
     return r.verdict != 0;
@@ -93,7 +95,7 @@ bool lucas_lehmer(unsigned long p) {
 
 bool lucas_lehmer_fft(unsigned long p) {
     bool v;
-    if (trivial(p, v)) return v;
+    if (trivial(p, v)) { g_last_res64 = 0; return v; }
     Run r = run_fft(p, 0);
     g_last_res64 = r.res64;
     return r.verdict != 0;
@@ -180,7 +182,7 @@ PYBIND11_MODULE(lucaslehmer, m, py::mod_gil_not_used()) {
     m.def("fft_res64_after",
           [](unsigned long p, unsigned long iters) {
               bool v;
-              if (trivial(p, v)) return std::make_pair(g_last_res64, 0.0);
+              if (trivial(p, v)) return std::make_pair(g_last_res64 = 0, 0.0);
               Run r = run_fft(p, iters ? iters : 0);
               g_last_res64 = r.res64;
               return std::make_pair(g_last_res64, r.maxerr);
@@ -190,7 +192,7 @@ PYBIND11_MODULE(lucaslehmer, m, py::mod_gil_not_used()) {
     m.def("gmp_res64_after",
           [](unsigned long p, unsigned long iters) {
               bool v;
-              if (trivial(p, v)) return g_last_res64;
+              if (trivial(p, v)) return g_last_res64 = 0;
               Run r = run_exact(p, iters);
               g_last_res64 = r.res64;
               return g_last_res64;

@@ -49,6 +49,8 @@ def lib():
         L.llh_ntt_length.restype = u64
         L.llh_max_exponent.argtypes = [ctypes.c_int]
         L.llh_max_exponent.restype = u64
+        L.llh_batch_max_exponent.argtypes = [ctypes.c_int]
+        L.llh_batch_max_exponent.restype = u64
         L.llh_set_fft_bits_per_word.argtypes = [dbl]
         L.llh_get_fft_bits_per_word.restype = dbl
         L.llh_set_batch_bits_per_word.argtypes = [dbl]
@@ -71,6 +73,9 @@ def lib():
         L.llh_mlp_predict.argtypes = [P(fl), P(fl), ctypes.c_size_t, P(fl)]
         L.llh_last_roundoff_trace.restype = ctypes.c_size_t
         L.llh_lucas_lehmer.argtypes = [u64, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32), P(i32)]
+        L.llh_last_run_stats.argtypes = [P(u64), P(u32), P(u64)]
+        L.llh_set_checkpoint_blocks.argtypes = [ctypes.c_int]
+        L.llh_relayout_host.argtypes = [u64, u64, P(ctypes.c_int64), u64, P(ctypes.c_int64)]
         _lib = L
     return _lib
 
@@ -246,3 +251,28 @@ def device_copy(nbytes: int, repeats: int = 1) -> float:
     ms = ctypes.c_double(0)
     _check(lib().llh_device_copy(nbytes, repeats, ctypes.byref(ms)))
     return ms.value
+
+
+def last_run_stats() -> dict:
+    """Large-engine squarings, rollbacks and the resume iteration of the last call."""
+    sq, rb, at = ctypes.c_uint64(0), ctypes.c_uint32(0), ctypes.c_uint64(0)
+    _check(lib().llh_last_run_stats(ctypes.byref(sq), ctypes.byref(rb), ctypes.byref(at)))
+    return {"large_squarings": sq.value, "rollbacks": rb.value, "resumed_at": at.value}
+
+
+def set_checkpoint_blocks(blocks: int) -> None:
+    _check(lib().llh_set_checkpoint_blocks(blocks))
+
+
+def get_checkpoint_blocks() -> int:
+    return lib().llh_get_checkpoint_blocks()
+
+
+def relayout_host(p: int, digits, n_to: int) -> np.ndarray:
+    """llh_relayout_host: canonical digits at len(digits) -> near-balanced digits at n_to."""
+    src = np.ascontiguousarray(np.asarray(digits, dtype=np.int64))
+    out = np.zeros(n_to, np.int64)
+    P = ctypes.POINTER
+    _check(lib().llh_relayout_host(p, len(src), src.ctypes.data_as(P(ctypes.c_int64)), n_to,
+                                   out.ctypes.data_as(P(ctypes.c_int64))))
+    return out

@@ -9,7 +9,7 @@ the low 64 bits of the canonical residue.  Each fixture is plain CSV:
   big_gmp_interim.csv    interim residues: p,iters,res64 (p = 1,048,573 @ 10,000;
                          p = 16,777,213 @ 5,000; p = 3,021,377 @ 20,000 = README.md:200-206)
   c4_exponents.csv       256 primes from [2^20, 2^25), numpy default_rng(0), with split
-  c4_gmp_heldout.csv     held-out subset: p,iters,res64 at 20,000 iterations
+  c4_gmp_heldout.csv     the 64 held-out exponents: p,iters,res64 at 20,000 iterations
   above_batch_gmp.csv    the four primes just above 21 x 8192 (beyond the batched FP64
                          lengths, so a batch call runs them on the large engine), full test:
                          p,prime,res64
@@ -86,18 +86,49 @@ def c4_list(path: str) -> None:
     print(f"{len(ps)} primes in [2^20, 2^25); wrote {len(pick)}")
 
 
-def c4_gmp(list_path: str, path: str, threads: int, count: int) -> None:
+def _read_rows(path: str) -> dict:
+    """{p: res64 hex} of a p,iters,res64 file (header optional); {} if absent."""
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            a = line.strip().split(",")
+            if len(a) == 3 and a[0].isdigit():
+                out[int(a[0])] = a[2]
+    return out
+
+
+def c4_gmp(list_path: str, path: str, threads: int, count: int, order: str, partial: str) -> None:
+    """GMP residues of the held-out exponents after 20,000 squarings, one exponent per thread.
+
+    Resumable: every finished exponent is appended to `partial` at once, and exponents already
+    in `path` or in any c4_gmp_heldout*.partial file beside it are skipped.  `order` asc starts
+    with the smallest exponents, desc with the largest (two machines can split the list).  When
+    every held-out exponent has a residue, `path` is rewritten sorted by p."""
+    import glob
+    from concurrent.futures import ThreadPoolExecutor, as_completed
     rows = [l.strip().split(",") for l in open(list_path)][1:]
     held = [int(p) for p, s in rows if s == "heldout"]
-    # an evenly spread subset of the held-out exponents (GMP at 2^25 bits is ~50 ms/squaring)
-    sub = held[:: max(1, len(held) // count)][:count]
+    done = _read_rows(path)
+    for f in glob.glob(os.path.join(os.path.dirname(path), "c4_gmp_heldout*.partial")) + [partial]:
+        done.update(_read_rows(f))
+    todo = sorted((p for p in held if p not in done), reverse=order == "desc")[:count]
     t0 = time.time()
-    out = O.ll_many(sub, iters=20000, engine=0, threads=threads)
-    with open(path, "w") as f:
-        f.write("p,iters,res64\n")
-        for p, (_, r) in zip(sub, out):
+    print(f"{len(done)} held-out residues known, {len(todo)} to run on {threads} threads", flush=True)
+    with open(partial, "a") as f, ThreadPoolExecutor(max_workers=threads) as ex:
+        futs = {ex.submit(O.ll_gmp, p, 20000): p for p in todo}   # ctypes drops the GIL
+        for fu in as_completed(futs):
+            p = futs[fu]
+            _, r, _ = fu.result()
             f.write(f"{p},20000,{r:016x}\n")
-    print(f"{len(sub)} held-out residues in {time.time() - t0:.0f} s")
+            f.flush()
+            done[p] = f"{r:016x}"
+            print(f"p={p} res64={r:016x} ({time.time() - t0:.0f} s)", flush=True)
+    if all(p in done for p in held):
+        with open(path, "w") as f:
+            f.write("p,iters,res64\n")
+            for p in held:
+                f.write(f"{p},20000,{done[p]}\n")
+        print(f"all {len(held)} held-out residues written to {path}", flush=True)
 
 
 def above_batch(path: str, threads: int) -> None:
@@ -115,7 +146,9 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above"])
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
-    ap.add_argument("--count", type=int, default=8)
+    ap.add_argument("--count", type=int, default=64)
+    ap.add_argument("--order", choices=["asc", "desc"], default="asc")
+    ap.add_argument("--partial", default=os.path.join(HERE, "c4_gmp_heldout.partial"))
     a = ap.parse_args()
     if a.what == "c1":
         run_range(10001, os.path.join(HERE, "c1_gmp_p10001.csv"), a.threads)
@@ -129,7 +162,7 @@ def main() -> None:
         c4_list(os.path.join(HERE, "c4_exponents.csv"))
     else:
         c4_gmp(os.path.join(HERE, "c4_exponents.csv"), os.path.join(HERE, "c4_gmp_heldout.csv"),
-               a.threads, a.count)
+               a.threads, a.count, a.order, a.partial)
 
 
 if __name__ == "__main__":

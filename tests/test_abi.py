@@ -24,13 +24,15 @@ def test_host_only_abi_calls():
     assert L.llh_ntt_length(100001) == 4096                   # cuda/ll_ntt.cu:176-180
     assert L.llh_batch_length(100001, llhip.NTT) == 4096
     assert L.llh_batch_length(9689, llhip.FP64) == 512        # 18.9 bits per digit <= 21
-    assert L.llh_max_exponent(llhip.NTT) == 4096 * 25
+    assert L.llh_max_exponent(llhip.NTT) == 8192 * 25           # cuda/ll_ntt.cu:279
+    assert L.llh_batch_max_exponent(llhip.NTT) == 4096 * 25
+    assert L.llh_batch_max_exponent(llhip.FP64) == 21 * 8192
     assert llhip.get_bpw() == 18.0 and llhip.get_batch_bpw() == 21.0
 
 
 def test_module_surface_and_host_functions():
     import lucaslehmer as ll
-    import lucaslehmer_hip as lh
+    import lucaslehmer_cuda as lc
     assert os.path.dirname(ll.__file__) == os.path.dirname(llhip.LIB_PATH)
     for name in ("lucas_lehmer", "lucas_lehmer_res", "lucas_lehmer_fft", "lucas_lehmer_fft_res",
                  "gmp_res64_after", "fft_res64_after", "fft_info", "set_fft_bits_per_word",
@@ -48,6 +50,22 @@ def test_module_surface_and_host_functions():
     assert ll.LucasLehmer().sa_getListOfPrimes(out, 2, 10) is True and out == [2, 3, 5, 7]
     # argument rules answered without a device call (lucaslehmer.cpp:30-32)
     assert ll.lucas_lehmer(2) is True and ll.lucas_lehmer(4) is False and ll.lucas_lehmer(1) is False
-    for name in ("lucas_lehmer_batch", "ntt_length", "max_exponent", "device_info"):
-        assert hasattr(lh, name)
-    assert lh.ntt_length(100001) == 4096
+    assert os.path.dirname(lc.__file__) == os.path.dirname(llhip.LIB_PATH)
+    for name in ("lucas_lehmer_batch", "ntt_length", "max_exponent", "device_info", "batch_length"):
+        assert hasattr(lc, name)
+
+
+def test_cuda_module_length_rule_equals_reference_rule():
+    """lucaslehmer_cuda.ntt_length and max_exponent are the reference's own (cuda/ll_ntt.cu:172-180,
+    :279), compared with the oracle's restatement at every p in [0, 210000]; host-only calls."""
+    import lucaslehmer_cuda as lc
+    from oracle import oracle as O
+    L = O.lib()
+    bad = [p for p in range(0, 210001) if lc.ntt_length(p) != L.orc_ntt_length(p)]
+    assert bad == []
+    assert lc.max_exponent() == 204800 and lc.ntt_length(204800) == 8192 and lc.ntt_length(204801) == 0
+    assert lc.ntt_length(100) == 64 and lc.ntt_length(0) == 64
+    # where the length steps up: N * max_bits(N), max_bits(N) = (63 - log2 N) // 2 worked by hand
+    for n, top in ((64, 1792), (128, 3584), (256, 6912), (512, 13824), (1024, 26624), (2048, 53248),
+                   (4096, 102400), (8192, 204800)):
+        assert lc.ntt_length(top) == n and lc.ntt_length(top + 1) == (2 * n if n < 8192 else 0)
