@@ -114,6 +114,18 @@ int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform,
 int llh_batch(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, int32_t *verdict,
               uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status);
 
+/* llh_batch for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and outputs as llh_batch; its device memory (the launch's items, results, class
+ * entries and queue counters, about 1.2 MB) is one file-scope array per GPU, and the batch tables
+ * are built by llh_init, so call llh_init first: the call then allocates nothing.  Limits: at most
+ * 16384 exponents (LLH_ERR_ARG beyond), and every p within the batched engine (FP64: p <= the
+ * batch cap * 8192; NTT: p <= 102,400; LLH_ERR_UNSUPPORTED otherwise, before any work).  Calls
+ * that overlapped on one GPU would race on the array; the library's lock serialises a process's
+ * calls, and each process has its own array. */
+int llh_batch_noalloc(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, int32_t *verdict,
+                      uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status);
+
 /* Device time of the kernel launches made by the last llh_batch / llh_lucas_lehmer call
  * (HIP events on the library's stream) and how many launches it took (1 + rollback launches).
  * No reference counterpart: measurement hook for the roofline report. */

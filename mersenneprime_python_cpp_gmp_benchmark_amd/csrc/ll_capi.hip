@@ -62,6 +62,9 @@ struct DeviceState {
     std::map<std::pair<int, int>, TableSet> tables;                   // (transform, class)
     void *d_items = nullptr, *d_out = nullptr, *d_cls = nullptr, *d_cnt = nullptr;
     size_t cap_items = 0, cap_cls = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evf[5] = {};              // batch launch timing / fork-join
+    unsigned char *scratch = nullptr;                                  // device address of `scratch`
+    std::string scratch_err;                                           // why it is null
 };
 
 struct LargeTables {
@@ -131,6 +134,28 @@ static const int kNttMaxLog = 12;                     // N = 16 .. 4096
         }                                                                                \
     } while (0)
 
+// Device memory of llh_batch_noalloc: the launch's items, results, class entries and queue
+// counters, in one file-scope array (each GPU the library runs on holds its own copy until the
+// process ends).  Host code reaches it only through the address hipGetSymbolAddress returns at
+// set-up (DeviceState::scratch), never through the name.
+constexpr size_t kScratchItems = 16384;          // exponents per call
+constexpr size_t kScratchClasses = 128;          // class entries per launch
+constexpr size_t kScratchItemBytes = kScratchItems * sizeof(BatchItem);
+constexpr size_t kScratchOutBytes = kScratchItems * sizeof(BatchResult);
+constexpr size_t kScratchClsBytes = kScratchClasses * sizeof(ClassDesc);
+constexpr size_t kScratchBytes = kScratchItemBytes + kScratchOutBytes + kScratchClsBytes + kScratchClasses * 4;
+static_assert(kScratchItemBytes % 256 == 0 && kScratchOutBytes % 256 == 0 && kScratchClsBytes % 256 == 0,
+              "scratch sections stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The per-class queue counters are cleared by this kernel rather than by hipMemsetAsync, so a
+// caller may capture the launch sequence in a graph.
+__global__ void fill_u32(u32 *p, u32 v, u32 n) {
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+static int build_batch_tables();
+
 static int init_locked(int device) {
     if (g_dev.id == device) return LLH_OK;
     if (g_dev.id >= 0) {
@@ -157,8 +182,22 @@ static int init_locked(int device) {
         g_dev.bpc[0][f] = batch_blocks_per_cu(kFP64, f);
         g_dev.bpc[1][f] = batch_blocks_per_cu(kNTT, f);
     }
+    HIPCHK(hipEventCreate(&g_dev.ev0));
+    HIPCHK(hipEventCreate(&g_dev.ev1));
+    for (int f = 0; f < 5; ++f) HIPCHK(hipEventCreateWithFlags(&g_dev.evf[f], hipEventDisableTiming));
+    // the array's address on this device; a failure disables llh_batch_noalloc only
+    void *sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se == hipSuccess) {
+        g_dev.scratch = (unsigned char *)sp;
+    } else {
+        g_dev.scratch = nullptr;
+        g_dev.scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+        (void)hipGetLastError();
+    }
     g_dev.id = device;
-    return LLH_OK;
+    // every batch class's tables, so that llh_batch_noalloc never has to allocate them
+    return build_batch_tables();
 }
 
 static int ensure_init() {
@@ -307,6 +346,17 @@ static const std::vector<std::pair<u64, int>> &batch_ladder(int transform) {
     return lad[transform == kNTT ? 1 : 0];
 }
 
+// Tables of every batch class of both transforms (the NTT runs the power-of-two classes only).
+static int build_batch_tables() {
+    for (int c = 1; c < batch_num_classes(); ++c) {
+        TableSet ts;
+        int rc = get_tables(kFP64, c, ts);
+        if (!rc && batch_class_pow2(c)) rc = get_tables(kNTT, c, ts);
+        if (rc) return rc;
+    }
+    return LLH_OK;
+}
+
 // the kernels index digits with 32-bit products j * p, j < N
 static bool batch_fits(u64 p, u64 N) { return (N + 2) * p < ((u64)1 << 32); }
 
@@ -378,7 +428,8 @@ struct Pending {
 
 // The persistent launches over `work` (one per class family, on separate streams so each
 // family's tail is back-filled by the others); fills res[i] for work[i].
-static int run_launch(int transform, std::vector<Pending> &work, std::vector<BatchResult> &res) {
+static int run_launch(int transform, std::vector<Pending> &work, std::vector<BatchResult> &res,
+                      bool noalloc = false) {
     const size_t n = work.size();
     res.assign(n, BatchResult());
     if (!n) return LLH_OK;
@@ -435,25 +486,42 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
         cls.back().count++;
     }
     int rc;
-    if ((rc = grow(&g_dev.d_items, g_dev.cap_items, n * sizeof(BatchItem) + n * sizeof(BatchResult) + 64)))
-        return rc;
-    if ((rc = grow(&g_dev.d_cls, g_dev.cap_cls, cls.size() * (sizeof(ClassDesc) + 16) + 64))) return rc;
-    BatchItem *d_items = (BatchItem *)g_dev.d_items;
-    BatchResult *d_out = (BatchResult *)((char *)g_dev.d_items + ((n * sizeof(BatchItem) + 63) / 64) * 64);
-    ClassDesc *d_cls = (ClassDesc *)g_dev.d_cls;
-    u32 *d_cnt = (u32 *)((char *)g_dev.d_cls + ((cls.size() * sizeof(ClassDesc) + 63) / 64) * 64);
+    BatchItem *d_items;
+    BatchResult *d_out;
+    ClassDesc *d_cls;
+    u32 *d_cnt;
+    if (noalloc) {                      // the file-scope array, through its device address
+        if (!g_dev.scratch) {
+            set_error("llh_batch_noalloc: " + g_dev.scratch_err);
+            return LLH_ERR_HIP;
+        }
+        if (n > kScratchItems || cls.size() > kScratchClasses) {
+            set_error("llh_batch_noalloc: more than 16384 exponents or 128 class entries in one launch");
+            return LLH_ERR_ARG;
+        }
+        d_items = (BatchItem *)g_dev.scratch;
+        d_out = (BatchResult *)(g_dev.scratch + kScratchItemBytes);
+        d_cls = (ClassDesc *)(g_dev.scratch + kScratchItemBytes + kScratchOutBytes);
+        d_cnt = (u32 *)(g_dev.scratch + kScratchItemBytes + kScratchOutBytes + kScratchClsBytes);
+    } else {
+        if ((rc = grow(&g_dev.d_items, g_dev.cap_items, n * sizeof(BatchItem) + n * sizeof(BatchResult) + 64)))
+            return rc;
+        if ((rc = grow(&g_dev.d_cls, g_dev.cap_cls, cls.size() * (sizeof(ClassDesc) + 16) + 64))) return rc;
+        d_items = (BatchItem *)g_dev.d_items;
+        d_out = (BatchResult *)((char *)g_dev.d_items + ((n * sizeof(BatchItem) + 63) / 64) * 64);
+        d_cls = (ClassDesc *)g_dev.d_cls;
+        d_cnt = (u32 *)((char *)g_dev.d_cls + ((cls.size() * sizeof(ClassDesc) + 63) / 64) * 64);
+    }
     hipStream_t st = g_dev.stream;
     HIPCHK(hipMemcpyAsync(d_items, items.data(), n * sizeof(BatchItem), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(ClassDesc), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(d_cnt, 0, cls.size() * sizeof(u32), st));
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, st, d_cnt, 0u, (u32)cls.size());
+    HIPCHK(hipGetLastError());
     const int tr = transform == kNTT ? 1 : 0;
     // kernel family f runs on stream f (the M = 4096 family shares the 3 * 2^k family's stream:
     // four streams fill the four hardware queues, and the cost table never picks 3 * 2^k there)
     constexpr int kFam = 5;
-    hipEvent_t ev0, ev1, evf[kFam];
-    HIPCHK(hipEventCreate(&ev0));
-    HIPCHK(hipEventCreate(&ev1));
-    for (int f = 0; f < kFam; ++f) HIPCHK(hipEventCreateWithFlags(&evf[f], hipEventDisableTiming));
+    hipEvent_t ev0 = g_dev.ev0, ev1 = g_dev.ev1, *evf = g_dev.evf;
     HIPCHK(hipEventRecord(ev0, st));
     bool joined[kFam] = {false, false, false, false, false};
     for (size_t i = 0; i < cls.size();) {
@@ -483,9 +551,6 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
     HIPCHK(hipStreamSynchronize(st));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    for (int f = 0; f < kFam; ++f) (void)hipEventDestroy(evf[f]);
     g_last_kernel_ms += ms;
     g_last_launches += 1;
     return LLH_OK;
@@ -1084,13 +1149,18 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
 
 static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int32_t *verdict,
                       uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status,
-                      u64 force_n = 0, bool no_rollback = false, const u64 *force_each = nullptr) {
+                      u64 force_n = 0, bool no_rollback = false, const u64 *force_each = nullptr,
+                      bool noalloc = false) {
     if (transform != kFP64 && transform != kNTT) {
         set_error("transform must be LLH_FP64 or LLH_NTT");
         return LLH_ERR_ARG;
     }
     if (n && (!ps || !verdict || !res64 || !status)) {
         set_error("NULL argument");
+        return LLH_ERR_ARG;
+    }
+    if (noalloc && n > kScratchItems) {
+        set_error("llh_batch_noalloc: at most 16384 exponents per call");
         return LLH_ERR_ARG;
     }
     std::lock_guard<std::recursive_mutex> lk(g_mx);
@@ -1138,12 +1208,17 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
         }
         work.push_back(Pending{p, it, i, batch_class_of(transform, bn), bn});
     }
+    if (noalloc && !large.empty()) {
+        set_error("llh_batch_noalloc: p = " + std::to_string(large[0].p) +
+                  " is beyond the batched engine (its device memory is allocated per call)");
+        return LLH_ERR_UNSUPPORTED;
+    }
     // exponents beyond the batched engine: the large engine, up to kLargeJobs at once (one per
     // stream), longest first; a job that rolls back restarts at the next length in its slot
     if ((rc = run_large_list(large, transform, no_rollback, verdict, res64, status, n_used, secs, errs))) return rc;
     while (!work.empty()) {
         std::vector<BatchResult> res;
-        rc = run_launch(transform, work, res);
+        rc = run_launch(transform, work, res, noalloc);
         if (rc) return rc;
         std::vector<Pending> again;
         for (size_t k = 0; k < work.size(); ++k) {
@@ -1294,6 +1369,17 @@ int llh_batch_n(const uint64_t *ps, size_t n, uint64_t max_iters, int transform,
     try {
         return batch_impl(ps, n, max_iters, transform, verdict, res64, seconds, maxerr, n_used, status, 0,
                           (flags & 1) != 0, n_force);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return LLH_ERR_HIP;
+    }
+}
+
+int llh_batch_noalloc(const uint64_t *ps, size_t n, uint64_t max_iters, int transform, int32_t *verdict,
+                      uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status) {
+    try {
+        return batch_impl(ps, n, max_iters, transform, verdict, res64, seconds, maxerr, n_used, status, 0, false,
+                          nullptr, true);
     } catch (const std::exception &e) {
         set_error(e.what());
         return LLH_ERR_HIP;
