@@ -131,19 +131,27 @@ int llh_batch_noalloc(const uint64_t *ps, size_t n, uint64_t max_iters, int tran
  * No reference counterpart: measurement hook for the roofline report. */
 int llh_last_launch_stats(double *kernel_ms, uint32_t *launches);
 
-/* Large-engine squarings queued by the last llh_batch / llh_lucas_lehmer call (those of a run
- * that rolled back included), how many rollbacks it took (both engines), and the iteration the
- * last rollback resumed from (0: from s_0).  No reference counterpart: the reference aborts at
- * round-off 0.35 and redoes the whole test with GMP (lucaslehmer.cpp:296, :331-335). */
-int llh_last_run_stats(uint64_t *large_squarings, uint32_t *rollbacks, uint64_t *resumed_at);
+/* Squarings the engines ran in the last llh_batch / llh_lucas_lehmer call (those of runs that
+ * rolled back included), how many rollbacks it took, how many of them resumed from a checkpoint,
+ * and the iteration the last of those resumed from.  No reference counterpart: the reference
+ * aborts at round-off 0.35 and redoes the whole test with GMP (lucaslehmer.cpp:296, :331-335). */
+int llh_last_run_stats(uint64_t *squarings, uint32_t *rollbacks, uint32_t *resumed, uint64_t *resumed_at);
 
-/* Checkpoint interval of the large FP64 engine in blocks of 200 squarings (default 1; 0 = a
- * rollback restarts from s_0).  After each such block the state is copied on the device; when a
- * later block's round-off exceeds 0.4, the last checkpoint whose blocks all stayed within 0.4 is
- * normalised on the device, re-split exactly at the next length, and the test resumes from its
- * iteration.  No reference counterpart (lucaslehmer.cpp:296 aborts the whole FP64 test). */
+/* Large-engine FP64 checkpoints (default 1; 0 = off, a rollback restarts from s_0): the state
+ * is copied on the device every `blocks` blocks of 200 squarings.  When the round-off exceeds
+ * 0.4, the last checkpoint reached within 0.4 is normalised on the device, re-split exactly at
+ * the next length, and the test resumes from its iteration.  No reference counterpart
+ * (lucaslehmer.cpp:296 aborts the whole FP64 test). */
 int llh_set_checkpoint_blocks(int blocks);
 int llh_get_checkpoint_blocks(void);
+
+/* Batch-engine FP64 checkpoints (default off): each exponent's digits are written at every
+ * round-off check (64 squarings), and an exponent that rolls back resumes at the next length
+ * from its last check within 0.4, re-split on the device.  Off, it restarts from s_0.  The
+ * checkpointing kernels run about 2% slower, and at the default bits-per-digit cap no batched
+ * exponent rolls back.  llh_batch_noalloc runs without them.  No reference counterpart. */
+int llh_set_batch_checkpoints(int on);
+int llh_get_batch_checkpoints(void);
 
 /* Host run of the per-digit arithmetic of the device re-layout (the checkpoint rollback's
  * N -> N' step): `from` holds the n_from canonical digits of X in [0, 2^p) (digit d has the

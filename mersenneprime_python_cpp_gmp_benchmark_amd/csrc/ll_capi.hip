@@ -34,6 +34,8 @@ hipError_t mlp_step(const MlpBuffers &B, float lr, int t, hipStream_t st);
 hipError_t launch_batch(int transform, int family, const ClassDesc *d_classes, int ncls, const BatchItem *d_items,
                         BatchResult *d_out, u32 *d_counters, int grid, hipStream_t stream);
 int batch_blocks_per_cu(int transform, int family);
+hipError_t batch_resume(int transform, const ResumeReq *d_reqs, int n, i64 *rb_area, i64 *resume, int *d_ok,
+                        hipStream_t st);
 
 bool large_launch_for(int transform, u64 M, LargeLaunch &L);
 int large_blocks(const LargeLaunch &L);
@@ -65,6 +67,10 @@ struct DeviceState {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evf[5] = {};              // batch launch timing / fork-join
     unsigned char *scratch = nullptr;                                  // device address of `scratch`
     std::string scratch_err;                                           // why it is null
+    // batch checkpoints: per-group slots (one region per family), rollback entries, resume
+    // entries, the rollback counter and the resume requests
+    void *ck = nullptr, *rb = nullptr, *res = nullptr, *rbc = nullptr, *req = nullptr;
+    size_t cap_ck = 0, cap_rb = 0, cap_res = 0, cap_rbc = 0, cap_req = 0;
 };
 
 struct LargeTables {
@@ -80,9 +86,10 @@ static std::map<std::pair<int, u64>, LargeTables> g_large_tables;
 static std::vector<double> g_err_trace;     // per-iteration max round-off of the last large run
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
-static u64 g_last_squarings = 0;          // squarings the large engine ran in the last call (kept or not)
+static u64 g_last_squarings = 0;          // squarings both engines ran in the last call (kept or not)
 static u32 g_last_rollbacks = 0;          // rollbacks in the last call
-static u64 g_last_resumed_at = 0;         // iteration the last rollback resumed from (0: from s_0)
+static u32 g_last_resumed = 0;            // of those, how many resumed from a checkpoint
+static u64 g_last_resumed_at = 0;         // iteration the last of them resumed from
 static std::atomic<double> g_bpw{18.0};        // reference knob: the conservative length rule
 // Bits-per-digit cap of the batched FP64 engine (lengths up to 8192).  The worst round-off over
 // every prime <= 100001 roughly doubles per half bit: 0.05 at 19.75-20, 0.10 at 20.25-20.5,
@@ -91,6 +98,12 @@ static std::atomic<double> g_bpw{18.0};        // reference knob: the conservati
 static std::atomic<double> g_batch_bpw{21.0};
 // Length families the batched engine may pick (bit f: family f; powers of two always).
 static std::atomic<int> g_batch_families{0xF};
+// Checkpoints.  The large engine copies its state every k graph blocks (0 = off: a rollback
+// restarts from s_0).  The batch engine's checkpoints (written at every round-off check) are
+// off by default: its checkpointing kernels run C3 2.3% slower, and at the default cap no batch
+// exponent rolls back.
+static std::atomic<int> g_ckpt_blocks{1};
+static std::atomic<int> g_batch_ckpt{0};
 
 // Measured cost of each batched length: device microseconds per exponent-squaring with the
 // GPU full of exponents of that length (the autotuner's table; ll_batch_costs.inc holds the
@@ -424,7 +437,13 @@ struct Pending {
     size_t idx;
     int cls;            // batched engine: size class (0 = the one-word lane, p < 64)
     u64 n = 0;          // the length (batched or large engine)
+    u64 it0 = 0;        // squarings done at the checkpoint it resumes from
+    u32 init = 0;       // resume entry + 1 (0: from s_0)
+    i64 rb = -1;        // rollback entry holding that checkpoint (before the re-layout)
+    u64 n_old = 0;      // the checkpoint's length
 };
+
+constexpr u32 kRollbackCap = 256;      // checkpointed rollbacks per launch (more restart from s_0)
 
 // The persistent launches over `work` (one per class family, on separate streams so each
 // family's tail is back-filled by the others); fills res[i] for work[i].
@@ -459,6 +478,8 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
         items[k].log_n = 0;
         items[k].iters = w.iters;
         items[k].slot = (u32)order[k];
+        items[k].it0 = (u32)w.it0;
+        items[k].init = w.init;
         items[k].pad = 0;
         // FP64 carry rounds (ll_batch.hip): one from log2 N + 3 bits per digit, else two.  Within
         // a class the items run p-descending, so each class splits into at most two entries.
@@ -480,6 +501,11 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
                 cd.tw = ts.tw; cd.twi = ts.twi; cd.wf = ts.wf; cd.wi = ts.wi;
                 cd.twp = ts.twp; cd.twpi = ts.twpi; cd.wtab = ts.wtab; cd.log_s = ts.log_s;
             }
+            cd.ck_slots = nullptr;
+            cd.rb_area = nullptr;
+            cd.rb_count = nullptr;
+            cd.rb_cap = 0;
+            cd.resume = (const i64 *)g_dev.res;
             cls.push_back(cd);
             fam_of.push_back(batch_class_family(w.cls));
         }
@@ -513,11 +539,32 @@ static int run_launch(int transform, std::vector<Pending> &work, std::vector<Bat
         d_cnt = (u32 *)((char *)g_dev.d_cls + ((cls.size() * sizeof(ClassDesc) + 63) / 64) * 64);
     }
     hipStream_t st = g_dev.stream;
+    const int tr = transform == kNTT ? 1 : 0;
+    // FP64 checkpoints: one slot region per kernel family (its largest grid x 16 WG digits)
+    if (transform == kFP64 && !noalloc && g_batch_ckpt.load()) {
+        size_t off[5], tot = 0;
+        for (int f = 0; f < 5; ++f) {
+            off[f] = tot;
+            tot += (size_t)g_dev.cus * (size_t)std::max(g_dev.bpc[tr][f], 1) * 16 * (size_t)batch_family_wg(f) * 8;
+        }
+        if ((rc = grow(&g_dev.ck, g_dev.cap_ck, tot)) ||
+            (rc = grow(&g_dev.rb, g_dev.cap_rb, (size_t)kRollbackCap * kResumeStride * 8)) ||
+            (rc = grow(&g_dev.rbc, g_dev.cap_rbc, 256)))
+            return rc;
+        for (size_t c = 0; c < cls.size(); ++c) {
+            if (!cls[c].cls) continue;
+            cls[c].ck_slots = (i64 *)((char *)g_dev.ck + off[fam_of[c]]);
+            cls[c].rb_area = (i64 *)g_dev.rb;
+            cls[c].rb_count = (u32 *)g_dev.rbc;
+            cls[c].rb_cap = kRollbackCap;
+        }
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, st, (u32 *)g_dev.rbc, 0u, 1u);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipMemcpyAsync(d_items, items.data(), n * sizeof(BatchItem), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(ClassDesc), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, st, d_cnt, 0u, (u32)cls.size());
     HIPCHK(hipGetLastError());
-    const int tr = transform == kNTT ? 1 : 0;
     // kernel family f runs on stream f (the M = 4096 family shares the 3 * 2^k family's stream:
     // four streams fill the four hardware queues, and the cost table never picks 3 * 2^k there)
     constexpr int kFam = 5;
@@ -701,7 +748,6 @@ struct Resume {
     u64 n = 0, done = 0;     // its length and the squarings it has done
 };
 
-static std::atomic<int> g_ckpt_blocks{1};   // checkpoint every k graph blocks; 0: restart from s_0
 
 struct LargeJob {
     struct Block {
@@ -762,8 +808,8 @@ struct LargeJob {
         const u64 nblk = (u64)large_blocks(L);
         errs_bytes = kGraphBlock * 8;
         eh_bytes = (kGraphBlock + 1) * 8;
-        // sync: the fused pass's launch counter, the hand-off fault word, one flag per column block
-        bytes_sync = 16 + nblk * 4;
+        // sync: the fused pass's launch counter, the hand-off fault word, its tagged carry words
+        bytes_sync = 16 + nblk * H * 8;
         bytes_elem = M * esz;
         bytes_carry = nblk * H * 8;
         bytes_dig = N * 8;
@@ -802,7 +848,7 @@ struct LargeJob {
         A.p_mod_n = (u32)(p % N); A.p_div_n = (u32)(p / N);
         A.ticket = (unsigned long long *)sync;
         A.fault = (unsigned *)((char *)sync + 8);
-        A.flags = (unsigned *)((char *)sync + 16);
+        A.tcarry = (char *)sync + 16;
         A.xcd_blocks = alone ? 1u : 0u;
         trace.clear();
         trace.reserve(iters - done);
@@ -946,7 +992,8 @@ struct LargeJob {
         }
         release();
         if (fault) {
-            set_error("large engine: a carry hand-off between workgroups timed out");
+            set_error("large engine: a carry hand-off between workgroups failed (a wait timed out, or a carry "
+                      "outgrew its 48-bit word)");
             return LLH_ERR_HIP;
         }
         return LLH_OK;
@@ -1119,7 +1166,10 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
                     secs[w.idx] += (double)(j.r.t1 - j.r.t0) / g_dev.wclk_hz;
                     errs[w.idx] = std::max(errs[w.idx], j.r.maxerr);
                     g_last_rollbacks += 1;
-                    g_last_resumed_at = ok ? from.done : 0;
+                    if (ok) {
+                        g_last_resumed += 1;
+                        g_last_resumed_at = from.done;
+                    }
                     rc = start(s, s.item, nx, ok ? &from : nullptr);
                     g_bufs.put(from.dig, from.bytes, false);
                     if (rc) return rc;
@@ -1147,6 +1197,45 @@ static int run_large_list(std::vector<Pending> &large, int transform, bool no_ro
     return LLH_OK;
 }
 
+// Re-splits the checkpoints of the rolled-back exponents at their next lengths (device kernel
+// bt_resume) into resume entries; the next launch starts each from its entry and its iteration.
+// A checkpoint whose normalisation does not settle starts again from s_0.
+static int resume_batch(int transform, std::vector<Pending> &again) {
+    std::vector<ResumeReq> reqs;
+    std::vector<size_t> which;
+    for (size_t k = 0; k < again.size(); ++k)
+        if (again[k].rb >= 0) {
+            reqs.push_back(ResumeReq{(u32)again[k].p, (u32)again[k].n_old, (u32)again[k].n, (u32)again[k].rb});
+            which.push_back(k);
+        }
+    if (reqs.empty()) return LLH_OK;
+    int rc;
+    const size_t nr = reqs.size();
+    if ((rc = grow(&g_dev.res, g_dev.cap_res, (size_t)kRollbackCap * kResumeStride * 8)) ||
+        (rc = grow(&g_dev.req, g_dev.cap_req, nr * (sizeof(ResumeReq) + 4) + 64)))
+        return rc;
+    hipStream_t st = g_dev.stream;
+    ResumeReq *d_req = (ResumeReq *)g_dev.req;
+    int *d_ok = (int *)((char *)g_dev.req + ((nr * sizeof(ResumeReq) + 63) / 64) * 64);
+    std::vector<int> ok(nr, 0);
+    HIPCHK(hipMemcpyAsync(d_req, reqs.data(), nr * sizeof(ResumeReq), hipMemcpyHostToDevice, st));
+    HIPCHK(batch_resume(transform, d_req, (int)nr, (i64 *)g_dev.rb, (i64 *)g_dev.res, d_ok, st));
+    HIPCHK(hipMemcpyAsync(ok.data(), d_ok, nr * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t e = 0; e < nr; ++e) {
+        Pending &w = again[which[e]];
+        if (ok[e]) {
+            w.init = (u32)e + 1;
+            g_last_resumed += 1;
+            g_last_resumed_at = w.it0;
+        } else {
+            w.it0 = 0;
+        }
+        w.rb = -1;
+    }
+    return LLH_OK;
+}
+
 static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int32_t *verdict,
                       uint64_t *res64, double *seconds, double *maxerr, uint32_t *n_used, int32_t *status,
                       u64 force_n = 0, bool no_rollback = false, const u64 *force_each = nullptr,
@@ -1170,6 +1259,7 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
     g_last_launches = 0;
     g_last_squarings = 0;
     g_last_rollbacks = 0;
+    g_last_resumed = 0;
     g_last_resumed_at = 0;
     std::vector<Pending> work, large;
     std::vector<double> secs(n, 0.0), errs(n, 0.0);
@@ -1226,10 +1316,17 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
             const BatchResult &r = res[k];
             secs[w.idx] += (double)(r.t1 - r.t0) / g_dev.wclk_hz;
             errs[w.idx] = std::max(errs[w.idx], r.maxerr);
+            if (r.iters_done > w.it0) g_last_squarings += r.iters_done - w.it0;
             if (r.status == kStRoundoff && w.cls > 0 && !no_rollback) {
-                const u64 nx = next_batch_length(w.p, w.n, transform);   // rollback: redo longer
+                const u64 nx = next_batch_length(w.p, w.n, transform);   // rollback: continue longer
                 if (nx) {
-                    again.push_back(Pending{w.p, w.iters, w.idx, batch_class_of(transform, nx), nx});
+                    Pending np{w.p, w.iters, w.idx, batch_class_of(transform, nx), nx};
+                    if (r.ck && nx <= kResumeStride) {        // from the last good checkpoint
+                        np.rb = (i64)r.ck - 1;
+                        np.it0 = r.ck_iter;
+                        np.n_old = w.n;
+                    }
+                    again.push_back(np);
                     g_last_rollbacks += 1;
                     continue;
                 }
@@ -1239,6 +1336,7 @@ static int batch_impl(const u64 *ps, size_t n, u64 max_iters, int transform, int
             status[w.idx] = r.status;
             if (n_used) n_used[w.idx] = (uint32_t)w.n;
         }
+        if ((rc = resume_batch(transform, again))) return rc;
         work.swap(again);
     }
     for (size_t i = 0; i < n; ++i) {
@@ -1547,10 +1645,11 @@ int llh_mlp_predict(const float *params, const float *X, size_t n, float *y_out)
     return LLH_OK;
 }
 
-int llh_last_run_stats(uint64_t *large_squarings, uint32_t *rollbacks, uint64_t *resumed_at) {
+int llh_last_run_stats(uint64_t *squarings, uint32_t *rollbacks, uint32_t *resumed, uint64_t *resumed_at) {
     std::lock_guard<std::recursive_mutex> lk(g_mx);
-    if (large_squarings) *large_squarings = g_last_squarings;
+    if (squarings) *squarings = g_last_squarings;
     if (rollbacks) *rollbacks = g_last_rollbacks;
+    if (resumed) *resumed = g_last_resumed;
     if (resumed_at) *resumed_at = g_last_resumed_at;
     return LLH_OK;
 }
@@ -1565,6 +1664,13 @@ int llh_set_checkpoint_blocks(int blocks) {
 }
 
 int llh_get_checkpoint_blocks(void) { return g_ckpt_blocks.load(); }
+
+int llh_set_batch_checkpoints(int on) {
+    g_batch_ckpt.store(on ? 1 : 0);
+    return LLH_OK;
+}
+
+int llh_get_batch_checkpoints(void) { return g_batch_ckpt.load(); }
 
 int llh_relayout_host(uint64_t p, uint64_t n_from, const int64_t *from, uint64_t n_to, int64_t *to) {
     if (!from || !to || p < 3 || n_from < 1 || n_to < 1 || n_from > p || n_to > p || n_from > ((u64)1 << 24) ||

@@ -246,3 +246,40 @@ struct FGL {
         return gl_mul_pow2<k>(a);
     }
 };
+
+// ---- exact re-layout of a residue from N digits to N' digits (checkpoint rollback) -------------
+// `dig` holds the N canonical digits of X in [0, 2^p): 0 <= dig[d] < 2^(c_{d+1} - c_d), with
+// c_d = ceil(d p / N) the position of digit d's lowest bit.  Digit k of the N'-digit form takes
+// bits [c'_k, c'_{k+1}) of X and is balanced by moving a carry of one into digit k + 1 when it is
+// at least half its range; the top digit's carry goes to digit 0 (2^p == 1).  Every digit of the
+// result is therefore in [-2^(b-1), 2^(b-1)], computed on its own (no carry chain).  The same
+// code runs on the device (the large engine's lg_relayout, the batch engine's bt_resume) and on
+// the host (llh_relayout_host, for tests).
+LL_HD u64 cpos(u64 d, u64 p, u64 n) { return (d * p + n - 1) / n; }   // ceil(d p / n), d p < 2^53
+
+// bits [lo, hi) of X (hi - lo < 64)
+LL_HD u64 bits_at(const i64 *dig, u64 p, u64 n, u64 lo, u64 hi) {
+    u64 v = 0;
+    u64 d = (lo * n) / p;               // c_d <= lo < c_{d+1}
+    u64 cd = cpos(d, p, n);
+    for (u64 a = lo; a < hi; ++d) {
+        const u64 cn = cpos(d + 1, p, n);
+        const u64 b = hi < cn ? hi : cn;
+        const u64 piece = ((u64)dig[d] >> (a - cd)) & ((1ull << (b - a)) - 1);
+        v |= piece << (a - lo);
+        a = b;
+        cd = cn;
+    }
+    return v;
+}
+
+LL_HD i64 relayout_digit(const i64 *dig, u64 p, u64 n, u64 n2, u64 k) {
+    const u64 km = k ? k - 1 : n2 - 1;
+    const u64 lo = cpos(k, p, n2), hi = cpos(k + 1, p, n2);
+    const u64 lo2 = cpos(km, p, n2), hi2 = cpos(km + 1, p, n2);
+    const int w = (int)(hi - lo), w2 = (int)(hi2 - lo2);
+    const u64 v = bits_at(dig, p, n, lo, hi), v2 = bits_at(dig, p, n, lo2, hi2);
+    const i64 c = (w > 0 && v >= (1ull << (w - 1))) ? 1 : 0;
+    const i64 c2 = (w2 > 0 && v2 >= (1ull << (w2 - 1))) ? 1 : 0;
+    return (i64)v - c * ((i64)1 << w) + c2;
+}

@@ -28,8 +28,10 @@ enum ItemStatus : int {
 struct BatchItem {
     u32 p;
     u32 log_n;
-    u64 iters;          // squarings to run (already min(p-2, cap))
+    u64 iters;          // squarings to run in total (already min(p-2, cap))
     u32 slot;           // index into the result array
+    u32 it0;            // squarings already done: the item resumes from a checkpoint
+    u32 init;           // resume entry + 1 (ClassDesc::resume, kResumeStride digits each); 0: s_0
     u32 pad;
 };
 
@@ -38,8 +40,19 @@ struct BatchResult {
     i32 status;         // ItemStatus
     u64 res64;          // low 64 bits of the canonical residue
     double maxerr;      // FP64: worst |v - rint(v)| over the run (0 for the exact transforms)
-    u64 iters_done;
+    u64 iters_done;     // squarings done when the item stopped
     u64 t0, t1;         // device wall clock at start / end of the item
+    u64 ck_iter;        // kStRoundoff: squarings done at the last good checkpoint
+    u32 ck;             // kStRoundoff: its rollback entry + 1 (ClassDesc::rb_area); 0: none
+    u32 pad;
+};
+
+// Digits per entry of the batch engine's rollback and resume areas (its longest length).
+constexpr u32 kResumeStride = 8192;
+
+// A batch exponent that continues at length n2 from rollback entry rb (its checkpoint at n).
+struct ResumeReq {
+    u32 p, n, n2, rb;
 };
 
 // One size class of a launch: items [first, first+count) share log_n and tables.
@@ -59,6 +72,16 @@ struct ClassDesc {
     i32 log_s;          // S = 2^log_s
     i32 cls;            // size class (ll_batch.hip); 0 = the one-word lane
     i32 carry2;         // FP64: two carry hand-off rounds (fewer than log2 N + 3 bits per digit)
+    // Checkpoints (FP64; all null: off).  At every round-off event that finds a group's run
+    // within the limit, the group writes its elements' raw words (natural order) to its slot,
+    // workgroup b at ck_slots + b * 16 * WG words, group g at + g * N; when an event finds the
+    // limit passed, the group copies the slot to entry atomicAdd(rb_count) < rb_cap of rb_area.
+    i64 *ck_slots;
+    i64 *rb_area;
+    u32 *rb_count;
+    u32 rb_cap;
+    u32 pad2;
+    const i64 *resume;  // resume entries of the items that continue from a checkpoint
 };
 constexpr int kWtabSlot = 128;
 
@@ -94,8 +117,8 @@ struct LargeArgs {
     double inv_n;                     // 1.0 / N
     u32 p_mod_n, p_div_n;             // p mod N, floor(p / N)
     unsigned long long *ticket;       // fused column pass: launch counter (only grows)
-    unsigned *flags;                  // [W / 4] per-block carry-out sequence numbers
-    unsigned *fault;                  // raised when a carry hand-off wait gives up
+    void *tcarry;                     // fused column pass: [nblk][H] tagged carry words
+    unsigned *fault;                  // raised when a carry hand-off fails (timeout, carry range)
     u32 xcd_blocks;                   // fused column pass: XCD-aware block order (job runs alone)
 };
 

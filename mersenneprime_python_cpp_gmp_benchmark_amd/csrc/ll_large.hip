@@ -127,6 +127,26 @@ LL_DEV double wt2d(const void *lo, const void *hi, u32 e, u32 log_s) {
     return ld_table(as_global<double>(lo), e & ((1u << log_s) - 1)) * ld_table(as_global<double>(hi), e >> log_s);
 }
 
+// A two-level table, entry e = lo[e mod 2^ls] * hi[e >> ls], read from global memory or from a
+// copy staged in LDS.
+LL_DEV C64 tmul(C64 a, C64 b) { return cmul(a, b); }
+LL_DEV u64 tmul(u64 a, u64 b) { return gl_mul(a, b); }
+LL_DEV double tmul(double a, double b) { return a * b; }
+template <class T, bool LDS>
+struct Tab2 {
+    const T *lo, *hi;
+    u32 ls;
+    LL_DEV T operator()(u32 e) const {
+        const u32 a = e & ((1u << ls) - 1), b = e >> ls;
+        if constexpr (LDS) return tmul(lo[a], hi[b]);
+        else return tmul(ld_table(as_global<T>(lo), a), ld_table(as_global<T>(hi), b));
+    }
+};
+template <class T>
+LL_HD Tab2<T, false> gtab(const void *lo, const void *hi, u32 ls) {
+    return Tab2<T, false>{(const T *)lo, (const T *)hi, ls};
+}
+
 // Digit layout for any N: c_d = ceil(d p / N), b_d = c_{d+1} - c_d, weight exponent
 // e_d = c_d N - d p in [0, N).  x = d p < 2^53, so one double-precision quotient estimate
 // is within one of floor(x / N) and a single correction makes it exact.
@@ -177,8 +197,7 @@ LL_DEV i64 split_i(i64 v, int b, i64 &c) {
 // ---- cross-workgroup hand-off ---------------------------------------------------------------
 // Values another workgroup reads inside the same launch are written and read with agent-scope
 // atomics, which go to the coherence point shared by all XCDs (the per-XCD L2s are not coherent
-// with each other inside a launch).  A workgroup publishes by completing those stores
-// (s_waitcnt vmcnt(0)), a barrier, then one flag store; a reader polls the flag, then reads.
+// with each other inside a launch).
 LL_DEV void st_coh(u64 *p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 LL_DEV u64 ld_coh_u(const u64 *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 LL_DEV void st_coh(double *p, double v) { st_coh((u64 *)p, (u64)__double_as_longlong(v)); }
@@ -190,33 +209,39 @@ LL_DEV void ld_coh(const i64 *p, i64 &v) { v = (i64)ld_coh_u((const u64 *)p); }
 // their "no wait" maxima, gfx9 encoding)
 LL_DEV void wait_vmem() { __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8)); }
 
-LL_DEV void publish_flag(unsigned *flag, u32 seq, int tid) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    wait_vmem();
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Tagged carry words of the fused column pass: bits 63..48 hold the launch's sequence number
+// (mod 2^16), bits 47..0 the signed carry.  Every launch rewrites every word, so a word holds
+// either this launch's value or the one before's, whose tag differs; a reader polls its own
+// words until the tag matches, with no flag and no barrier between publishing and reading.
+constexpr u64 kTagShift = 48;
+constexpr i64 kCarryLim = (i64)1 << 47;
+LL_DEV u64 tag_word(u32 seq, i64 c) {
+    return ((u64)(seq & 0xFFFFu) << kTagShift) | ((u64)c & ((1ull << kTagShift) - 1));
 }
+LL_DEV i64 untag(u64 w) { return (i64)(w << 16) >> 16; }
+LL_DEV i64 carry_i(double c) { return (i64)c; }      // integral, |c| < 2^53
+LL_DEV i64 carry_i(i64 c) { return c; }
+LL_DEV void carry_from(i64 v, double &c) { c = (double)v; }
+LL_DEV void carry_from(i64 v, i64 &c) { c = v; }
 
-// Thread 0 polls until flag >= seq (sequence numbers, wrap-safe compare), for at most about
-// two seconds of wall clock; past that it raises *fault and stops waiting, so the launch always
-// drains (and once *fault is up, later waits give up at once).  The whole workgroup leaves
-// together.
+// Polls until the word at p carries tag seq, for at most about two seconds of wall clock; past
+// that it raises *fault and stops waiting, so the launch always drains (and once *fault is up,
+// later waits give up within a few polls).
 constexpr u64 kSpinTicks = 200000000ull;   // 2 s at the usual 100 MHz wall clock
-LL_DEV void wait_flag(const unsigned *flag, u32 seq, unsigned *fault, int tid) {
-    if (tid == 0) {
-        const u64 t0 = wall_clock64();
-        while ((int)(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - seq) < 0) {
-            if (wall_clock64() - t0 > kSpinTicks ||
-                __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                atomicOr(fault, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
+LL_DEV u64 wait_tagged(const u64 *p, u32 seq, unsigned *fault) {
+    u64 w = ld_coh_u(p);
+    if ((u32)(w >> kTagShift) == (seq & 0xFFFFu)) return w;
+    const u64 t0 = wall_clock64();
+    for (u32 k = 1;; ++k) {
+        __builtin_amdgcn_s_sleep(1);
+        w = ld_coh_u(p);
+        if ((u32)(w >> kTagShift) == (seq & 0xFFFFu)) return w;
+        if (wall_clock64() - t0 > kSpinTicks ||
+            ((k & 63) == 0 && __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            atomicOr(fault, 1u);
+            return w & ~((1ull << kTagShift) - 1);    // carry 0: the run is reported as failed
         }
     }
-    wait_vmem();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    __syncthreads();
 }
 
 // Per-pass twiddle tables of the H-point and W-point transforms, copied into LDS out of the full
@@ -256,20 +281,20 @@ struct Col {
     static_assert(H == NT * NROW, "row segments must split evenly over the workgroup");
 };
 
-// w_M^{w k_h} at the thread's positions k_h = t + GH q (tables lo/hi: forward or GL inverse)
-template <bool IS_GL, int W, int H>
-LL_DEV void col_twiddles(typename Col<IS_GL, W, H>::T (&tm)[kR], const void *lo, const void *hi, u32 log_s,
-                         int w, int t) {
+// w_M^{w k_h} at the thread's positions k_h = t + GH q (table: forward or GL inverse); the
+// exponent w k_h < W H = M needs no reduction
+template <bool IS_GL, int W, int H, class TAB>
+LL_DEV void col_twiddles(typename Col<IS_GL, W, H>::T (&tm)[kR], const TAB &tab, int w, int t) {
     typedef Col<IS_GL, W, H> C;
 #pragma unroll
-    for (int q = 0; q < kR; ++q) tm[q] = tw2<typename C::T>(lo, hi, (u32)w * (u32)(t + C::GH * q), log_s);
+    for (int q = 0; q < kR; ++q) tm[q] = tab((u32)w * (u32)(t + C::GH * q));
 }
 
-// Digit weights (tables lo/hi: forward or inverse) at the thread's elements
-// j = w + W (t + GH q); FP64 elements hold digits 2j and 2j + 1.
-template <bool IS_GL, int W, int H>
-LL_DEV void col_weights(typename Col<IS_GL, W, H>::WT (&wt)[nwt(IS_GL)], const void *lo,
-                        const void *hi, const LargeArgs &A, int w, int t) {
+// Digit weights (table: forward or inverse) at the thread's elements j = w + W (t + GH q);
+// FP64 elements hold digits 2j and 2j + 1.
+template <bool IS_GL, int W, int H, class TAB>
+LL_DEV void col_weights(typename Col<IS_GL, W, H>::WT (&wt)[nwt(IS_GL)], const TAB &tab, const LargeArgs &A,
+                        int w, int t) {
     typedef Col<IS_GL, W, H> C;
     const ExpWalk EW = ewalk_of(A);
     const NDiv D = ndiv_of(A);
@@ -279,10 +304,10 @@ LL_DEV void col_weights(typename Col<IS_GL, W, H>::WT (&wt)[nwt(IS_GL)], const v
 #pragma unroll
     for (int q = 0; q < kR; ++q) {
         if constexpr (IS_GL) {
-            wt[q] = tw2<u64>(lo, hi, e, A.log_sw);
+            wt[q] = tab(e);
         } else {
-            wt[2 * q] = wt2d(lo, hi, e, A.log_sw);
-            wt[2 * q + 1] = wt2d(lo, hi, EW.next(e), A.log_sw);
+            wt[2 * q] = tab(e);
+            wt[2 * q + 1] = tab(EW.next(e));
         }
         e = EW.sub(e, step);
     }
@@ -474,9 +499,9 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fwd(LargeArgs A) {
         cin[i] = blk > 0 ? carry[(blk - 1) * H + h] : carry[(C::NBLK - 1) * H + (h > 0 ? h - 1 : H - 1)];
     }
     WT wf[C::NWT];
-    col_weights<IS_GL, W, H>(wf, A.wfL, A.wfH, A, w, t);
+    col_weights<IS_GL, W, H>(wf, gtab<WT>(A.wfL, A.wfH, A.log_sw), A, w, t);
     T tm[kR];
-    col_twiddles<IS_GL, W, H>(tm, A.tmL, A.tmH, A.log_s, w, t);
+    col_twiddles<IS_GL, W, H>(tm, gtab<T>(A.tmL, A.tmH, A.log_s), w, t);
     stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
 #pragma unroll
     for (int s = 0; s < kR; ++s) lds_all[(t + C::GH * s) * kCW + c] = x[s];
@@ -498,11 +523,11 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
     const int blk = blockIdx.x, w = blk * kCW + c;
     if (tid == 0) s_err = 0;
     T tmi[kR];
-    col_twiddles<IS_GL, W, H>(tmi, IS_GL ? A.tmLi : A.tmL, IS_GL ? A.tmHi : A.tmH, A.log_s, w, t);
+    col_twiddles<IS_GL, W, H>(tmi, gtab<T>(IS_GL ? A.tmLi : A.tmL, IS_GL ? A.tmHi : A.tmH, A.log_s), w, t);
     T x[kR];
     col_load_inv<IS_GL, W, H>(x, tmi, A, w, t);
     WT wi[C::NWT];
-    col_weights<IS_GL, W, H>(wi, A.wiL, A.wiH, A, w, t);
+    col_weights<IS_GL, W, H>(wi, gtab<WT>(A.wiL, A.wiH, A.log_sw), A, w, t);
     stage_col_table<T, H, C::NT>(s_tw, IS_GL ? A.twHi : A.twH, tid);
     CT cy[C::NROW];
     double eloc = 0.0;
@@ -521,13 +546,18 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_inv(LargeArgs A) {
 }
 
 // ---- K3+K1 fused: inverse column pass of one squaring and forward column pass of the next --------
-// Workgroup b runs the inverse pass of block b, publishes its row-segment carry-outs, waits for
-// block b - 1's (block 0: block NBLK - 1's, one row down, the top one wrapping to digit 0), adds
-// them and runs the forward pass.  Every workgroup publishes before it waits, and b >= 1 waits on
-// b - 1, which the in-order dispatch of workgroups has already placed; block 0 waits on the last
-// block, which needs only one free CU once the others finish.  Flags hold launch sequence
-// numbers: every workgroup adds one to a counter that only grows, so ticket / NBLK + 1 is the
-// same for all workgroups of one launch and larger than any earlier launch's.
+// Workgroup b runs the inverse pass of block b, publishes its row-segment carry-outs as tagged
+// words, waits for block b - 1's (block 0: block NBLK - 1's, one row down, the top one wrapping
+// to digit 0), adds them and runs the forward pass.  Progress: every workgroup publishes before
+// it waits, so a waiting workgroup only needs its predecessor to be resident or dispatched
+// later.  With the plain order, b waits on b - 1, which in-order dispatch has already placed,
+// and block 0 on the last block, which needs one free CU once the others finish.  With the
+// XCD-aware order (below) the predecessor of the workgroup at blockIdx x < 8 is dispatched near
+// the end; the launch still drains as long as 9 workgroups fit on the GPU at once, because
+// every workgroup that waits has published and the predecessors it waits on only wait on
+// published blocks.  The tag is the launch's sequence number: every workgroup adds one to a
+// counter that only grows, so ticket / NBLK + 1 is the same for all workgroups of one launch
+// and one more than the launch before's.
 // Column block of workgroup b in the fused pass.  Workgroups are dealt to the 8 XCDs in turn
 // (b mod 8); mapping runs of NBLK / 8 consecutive blocks to one XCD keeps each block's
 // predecessor, whose carries and flag it waits for, on the same XCD for all but 8 hand-offs,
@@ -561,35 +591,38 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fused(LargeArgs A) {
     stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
     if constexpr (IS_GL) stage_col_table<T, H, C::NT>(s_twi, A.twHi, tid);
     T tm[kR];
-    col_twiddles<IS_GL, W, H>(tm, A.tmL, A.tmH, A.log_s, w, t);
+    col_twiddles<IS_GL, W, H>(tm, gtab<T>(A.tmL, A.tmH, A.log_s), w, t);
     T x[kR];
     if constexpr (IS_GL) {
         T tmi[kR];
-        col_twiddles<IS_GL, W, H>(tmi, A.tmLi, A.tmHi, A.log_s, w, t);
+        col_twiddles<IS_GL, W, H>(tmi, gtab<T>(A.tmLi, A.tmHi, A.log_s), w, t);
         col_load_inv<IS_GL, W, H>(x, tmi, A, w, t);
     } else {
         col_load_inv<IS_GL, W, H>(x, tm, A, w, t);
     }
     WT wi[C::NWT], wf[C::NWT];
-    col_weights<IS_GL, W, H>(wi, A.wiL, A.wiH, A, w, t);
-    col_weights<IS_GL, W, H>(wf, A.wfL, A.wfH, A, w, t);
+    col_weights<IS_GL, W, H>(wi, gtab<WT>(A.wiL, A.wiH, A.log_sw), A, w, t);
+    col_weights<IS_GL, W, H>(wf, gtab<WT>(A.wfL, A.wfH, A.log_sw), A, w, t);
     CT cy[C::NROW];
     double eloc = 0.0;
     col_inverse<IS_GL, W, H>(x, lds_all, (const T *)(IS_GL ? s_twi : s_tw), wi, A, blk, c, t, tid, cy, eloc);
     const u32 seq = s_seq;   // written before the barriers inside col_inverse
-    CT *carry = reinterpret_cast<CT *>(A.carry);
+    u64 *tc = reinterpret_cast<u64 *>(A.tcarry);
+    bool wide = false;
 #pragma unroll
-    for (int i = 0; i < C::NROW; ++i) st_coh(carry + blk * H + tid + C::NT * i, cy[i]);
-    publish_flag(A.flags + blk, seq, tid);
+    for (int i = 0; i < C::NROW; ++i) {
+        const i64 ci = carry_i(cy[i]);
+        wide |= ci >= kCarryLim || ci < -kCarryLim;
+        st_coh(tc + blk * H + tid + C::NT * i, tag_word(seq, ci));
+    }
+    if (wide) atomicOr(A.fault, 2u);            // a carry the tagged word cannot hold: fail the run
     if constexpr (!IS_GL) atomicMax(&s_err, (unsigned long long)__double_as_longlong(eloc));
-    const int prev = blk > 0 ? blk - 1 : NBLK - 1;
-    wait_flag(A.flags + prev, seq, A.fault, tid);
     CT cin[C::NROW];
 #pragma unroll
     for (int i = 0; i < C::NROW; ++i) {
         const int h = tid + C::NT * i;
         const int src = blk > 0 ? (blk - 1) * H + h : (NBLK - 1) * H + (h > 0 ? h - 1 : H - 1);
-        ld_coh(carry + src, cin[i]);
+        carry_from(untag(wait_tagged(tc + src, seq, A.fault)), cin[i]);
     }
     col_forward<IS_GL, W, H>(lds_all, (const T *)s_tw, wf, tm, A, blk, c, t, tid, cin);
     // the workgroup's round-off maximum (s_err, complete after the barriers of col_forward) goes
@@ -796,43 +829,7 @@ __global__ void __launch_bounds__(kNormThreads) lg_normalize(const void *elem_v,
 
 }  // namespace
 
-// ---- exact re-layout of a residue from N digits to N' digits (checkpoint rollback) -------------
-// `dig` holds the N canonical digits of X in [0, 2^p): 0 <= dig[d] < 2^(c_{d+1} - c_d), with
-// c_d = ceil(d p / N) the position of digit d's lowest bit.  Digit k of the N'-digit form takes
-// bits [c'_k, c'_{k+1}) of X and is balanced by moving a carry of one into digit k + 1 when it is
-// at least half its range; the top digit's carry goes to digit 0 (2^p == 1).  Every digit of the
-// result is therefore in [-2^(b-1), 2^(b-1)], computed on its own (no carry chain).  The same
-// code runs on the device (lg_relayout) and on the host (llh_relayout_host, for tests).
 namespace {
-
-LL_HD u64 cpos(u64 d, u64 p, u64 n) { return (d * p + n - 1) / n; }   // ceil(d p / n), d p < 2^53
-
-// bits [lo, hi) of X (hi - lo < 64)
-LL_HD u64 bits_at(const i64 *dig, u64 p, u64 n, u64 lo, u64 hi) {
-    u64 v = 0;
-    u64 d = (lo * n) / p;               // c_d <= lo < c_{d+1}
-    u64 cd = cpos(d, p, n);
-    for (u64 a = lo; a < hi; ++d) {
-        const u64 cn = cpos(d + 1, p, n);
-        const u64 b = hi < cn ? hi : cn;
-        const u64 piece = ((u64)dig[d] >> (a - cd)) & ((1ull << (b - a)) - 1);
-        v |= piece << (a - lo);
-        a = b;
-        cd = cn;
-    }
-    return v;
-}
-
-LL_HD i64 relayout_digit(const i64 *dig, u64 p, u64 n, u64 n2, u64 k) {
-    const u64 km = k ? k - 1 : n2 - 1;
-    const u64 lo = cpos(k, p, n2), hi = cpos(k + 1, p, n2);
-    const u64 lo2 = cpos(km, p, n2), hi2 = cpos(km + 1, p, n2);
-    const int w = (int)(hi - lo), w2 = (int)(hi2 - lo2);
-    const u64 v = bits_at(dig, p, n, lo, hi), v2 = bits_at(dig, p, n, lo2, hi2);
-    const i64 c = (w > 0 && v >= (1ull << (w - 1))) ? 1 : 0;
-    const i64 c2 = (w2 > 0 && v2 >= (1ull << (w2 - 1))) ? 1 : 0;
-    return (i64)v - c * ((i64)1 << w) + c2;
-}
 
 // One thread per output digit; the output is the element array of the N'-digit engine (FP64:
 // N' doubles, element j = digits 2j, 2j + 1; GL: N' words holding the signed digit).

@@ -73,7 +73,7 @@ def lib():
         L.llh_mlp_predict.argtypes = [P(fl), P(fl), ctypes.c_size_t, P(fl)]
         L.llh_last_roundoff_trace.restype = ctypes.c_size_t
         L.llh_lucas_lehmer.argtypes = [u64, u64, ctypes.c_int, P(i32), P(u64), P(dbl), P(u32), P(i32)]
-        L.llh_last_run_stats.argtypes = [P(u64), P(u32), P(u64)]
+        L.llh_last_run_stats.argtypes = [P(u64), P(u32), P(u32), P(u64)]
         L.llh_set_checkpoint_blocks.argtypes = [ctypes.c_int]
         L.llh_relayout_host.argtypes = [u64, u64, P(ctypes.c_int64), u64, P(ctypes.c_int64)]
         _lib = L
@@ -254,10 +254,10 @@ def device_copy(nbytes: int, repeats: int = 1) -> float:
 
 
 def last_run_stats() -> dict:
-    """Large-engine squarings, rollbacks and the resume iteration of the last call."""
-    sq, rb, at = ctypes.c_uint64(0), ctypes.c_uint32(0), ctypes.c_uint64(0)
-    _check(lib().llh_last_run_stats(ctypes.byref(sq), ctypes.byref(rb), ctypes.byref(at)))
-    return {"large_squarings": sq.value, "rollbacks": rb.value, "resumed_at": at.value}
+    """Squarings run, rollbacks, checkpoint resumes and the last resume iteration of the last call."""
+    sq, rb, rs, at = ctypes.c_uint64(0), ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint64(0)
+    _check(lib().llh_last_run_stats(ctypes.byref(sq), ctypes.byref(rb), ctypes.byref(rs), ctypes.byref(at)))
+    return {"squarings": sq.value, "rollbacks": rb.value, "resumed": rs.value, "resumed_at": at.value}
 
 
 def set_checkpoint_blocks(blocks: int) -> None:
@@ -266,6 +266,14 @@ def set_checkpoint_blocks(blocks: int) -> None:
 
 def get_checkpoint_blocks() -> int:
     return lib().llh_get_checkpoint_blocks()
+
+
+def set_batch_checkpoints(on: bool) -> None:
+    _check(lib().llh_set_batch_checkpoints(1 if on else 0))
+
+
+def get_batch_checkpoints() -> bool:
+    return bool(lib().llh_get_batch_checkpoints())
 
 
 def relayout_host(p: int, digits, n_to: int) -> np.ndarray:

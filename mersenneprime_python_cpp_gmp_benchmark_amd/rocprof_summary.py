@@ -3,6 +3,7 @@
     python -m mersenneprime_python_cpp_gmp_benchmark_amd.rocprof_summary DB "COMMAND" [OUT.md]
     python -m mersenneprime_python_cpp_gmp_benchmark_amd.rocprof_summary --traffic FETCH_DB WRITE_DB \
         COPY_FETCH_DB COPY_WRITE_DB COPY_BYTES
+    python -m mersenneprime_python_cpp_gmp_benchmark_amd.rocprof_summary --counters MATCH DB [DB ...]
 
 Reads the ``kernels`` view (one row per dispatch) and prints, per kernel name: calls,
 total / average / min / max duration, grid and workgroup size, VGPR/AGPR/SGPR counts and
@@ -67,7 +68,36 @@ def traffic(fetch_db: str, write_db: str, copy_fetch_db: str, copy_write_db: str
     return {"fetch_scale": fs, "write_scale": ws, "kernels": rows}
 
 
+def counters(dbs: list, match: str = "") -> str:
+    """Markdown table: per kernel (name containing `match`), the average per dispatch of every PMC
+    counter found in the given databases (one counter set per rocprofv3 --pmc pass)."""
+    table = defaultdict(dict)
+    calls = {}
+    for db in dbs:
+        c = sqlite3.connect(db)
+        acc = defaultdict(lambda: defaultdict(list))
+        for name, cn, v in c.execute("select kernel_name, counter_name, value from counters_collection"):
+            if match in name:
+                acc[name][cn].append(float(v))
+        for name, d in acc.items():
+            for cn, vals in d.items():
+                table[name][cn] = sum(vals) / len(vals)
+                calls[name] = len(vals)
+    names = sorted(table, key=lambda n: -calls[n])
+    cols = sorted({cn for n in names for cn in table[n]})
+    short = [n.replace("void (anonymous namespace)::", "").split("(")[0] for n in names]
+    out = ["| counter (average per dispatch) | " + " | ".join(f"`{s}`" for s in short) + " |",
+           "|---|" + "---|" * len(names), "| dispatches | " + " | ".join(str(calls[n]) for n in names) + " |"]
+    for cn in cols:
+        out.append(f"| {cn} | " + " | ".join(f"{table[n].get(cn, float('nan')):.4g}" for n in names) + " |")
+    return "\n".join(out) + "\n"
+
+
 def main() -> int:
+    if len(sys.argv) > 1 and sys.argv[1] == "--counters":
+        # --counters MATCH DB [DB ...]
+        print(counters(sys.argv[3:], sys.argv[2]))
+        return 0
     if len(sys.argv) > 1 and sys.argv[1] == "--traffic":
         # --traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB COPY_BYTES
         t = traffic(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], int(sys.argv[6]))

@@ -10,11 +10,12 @@ the low 64 bits of the canonical residue.  Each fixture is plain CSV:
                          p = 16,777,213 @ 5,000; p = 3,021,377 @ 20,000 = README.md:200-206)
   c4_exponents.csv       256 primes from [2^20, 2^25), numpy default_rng(0), with split
   c4_gmp_heldout.csv     the 64 held-out exponents: p,iters,res64 at 20,000 iterations
+  rollback_gmp.csv       p,iters,res64 of the large engine's rollback cases at 6,000 squarings
   above_batch_gmp.csv    the four primes just above 21 x 8192 (beyond the batched FP64
                          lengths, so a batch call runs them on the large engine), full test:
                          p,prime,res64
 
-Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp|above [--threads T]
+Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp|above|rollback [--threads T]
 """
 from __future__ import annotations
 
@@ -131,6 +132,19 @@ def c4_gmp(list_path: str, path: str, threads: int, count: int, order: str, part
         print(f"all {len(held)} held-out residues written to {path}", flush=True)
 
 
+def rollback(path: str, threads: int) -> None:
+    """GMP residues of the large engine's rollback cases after 6,000 squarings: exponents whose
+    FP64 round-off first exceeds 0.4 mid-run at a forced length (p = 1,333,663 at N = 65,536,
+    p = 1,658,893 at N = 81,920)."""
+    cases = [1333663, 1658893]
+    out = O.ll_many(cases, iters=6000, engine=0, threads=threads)
+    with open(path, "w") as f:
+        f.write("p,iters,res64\n")
+        for p, (_, r) in zip(cases, out):
+            f.write(f"{p},6000,{r:016x}\n")
+    print(f"{len(cases)} rollback residues written")
+
+
 def above_batch(path: str, threads: int) -> None:
     ps = O.primes(21 * 8192 + 1, 21 * 8192 + 200)[:4]
     t0 = time.time()
@@ -144,7 +158,7 @@ def above_batch(path: str, threads: int) -> None:
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above"])
+    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above", "rollback"])
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--order", choices=["asc", "desc"], default="asc")
@@ -158,6 +172,8 @@ def main() -> None:
         run_big(os.path.join(HERE, "big_gmp_interim.csv"))
     elif a.what == "above":
         above_batch(os.path.join(HERE, "above_batch_gmp.csv"), a.threads)
+    elif a.what == "rollback":
+        rollback(os.path.join(HERE, "rollback_gmp.csv"), a.threads)
     elif a.what == "c4list":
         c4_list(os.path.join(HERE, "c4_exponents.csv"))
     else:

@@ -159,3 +159,35 @@ def test_carry_rounds_both_sides_of_threshold():
         for i, p in enumerate(ps):
             assert int(out["res64"][i]) == O.ll_gmp(p, 2000)[1], (p, n)
         assert float(np.max(out["maxerr"])) < 0.1, (n, list(out["maxerr"]))
+
+
+def _c3_subset():
+    rows = load_range_fixture("c3_gmp_p100001.csv")
+    return [r for r in rows if r[0] > 20000][::6]
+
+
+def test_rollbacks_resume_from_checkpoints():
+    """At 22 bits per digit the batch lengths' round-off crosses 0.4 mid-run for some exponents.
+    Each must roll back to the next length from its last good checkpoint (at most one check
+    interval, 64 squarings, redone), not from s_0, and every Res64 must equal GMP's."""
+    sel = _c3_subset()
+    ps = [r[0] for r in sel]
+    total = sum(p - 2 for p in ps)
+    old = llhip.get_batch_bpw()
+    assert not llhip.get_batch_checkpoints()
+    try:
+        llhip.set_batch_bpw(22.0)
+        llhip.set_batch_checkpoints(True)
+        out = llhip.batch(ps, 0, llhip.FP64)
+        st = llhip.last_run_stats()
+        llhip.set_batch_checkpoints(False)
+        out0 = llhip.batch(ps, 0, llhip.FP64)
+        st0 = llhip.last_run_stats()
+    finally:
+        llhip.set_batch_checkpoints(False)
+        llhip.set_batch_bpw(old)
+    _check_rows(sel, out)
+    _check_rows(sel, out0)
+    assert st["rollbacks"] > 0 and st["resumed"] == st["rollbacks"], st
+    assert total <= st["squarings"] <= total + 64 * st["rollbacks"], (st, total)
+    assert st0["rollbacks"] > 0 and st0["resumed"] == 0 and st0["squarings"] > st["squarings"]

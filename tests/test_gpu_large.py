@@ -63,12 +63,15 @@ def test_composite_full_test_matches_gmp():
 
 
 def test_interim_fixtures():
+    """Every committed interim fixture on both transforms at the default lengths: p = 3,021,377
+    @ 20,000 (README.md:200-206), p = 1,048,573 @ 10,000 (the plain GPU form's configuration)
+    and p = 16,777,213 @ 5,000 (the exact NTT configuration, N = 2^20)."""
     for p, k, res in load_interim_fixture("big_gmp_interim.csv"):
-        if p > 4_000_000:
-            continue           # the p = 2^24 case runs in the bench
         for tr in (llhip.FP64, llhip.NTT):
             r = llhip.lucas_lehmer(p, k, tr)
             assert r["status"] == llhip.ST_OK and r["res64"] == res, (tr, p, k, r)
+            if p == 16777213:
+                assert r["n_used"] == 1 << 20
 
 
 def test_default_length_routes_large_exponents():
@@ -127,3 +130,40 @@ def test_concurrent_large_jobs_equal_single_runs(tr):
         assert one["status"] == llhip.ST_OK and int(out["status"][i]) == llhip.ST_OK, p
         assert int(out["res64"][i]) == one["res64"], p
         assert int(out["n_used"][i]) == one["n_used"], p
+
+
+ROLLBACK = [(1333663, 1 << 16, 81920), (1658893, 81920, 98304)]   # p, forced length, next length
+
+
+@pytest.mark.parametrize("p,n,n_next", ROLLBACK, ids=[f"p{c[0]}-N{c[1]}" for c in ROLLBACK])
+def test_rollback_resumes_from_checkpoint_at_next_length(p, n, n_next):
+    """At this forced length the FP64 round-off first exceeds 0.4 mid-run (a no-rollback pass
+    finds where); with rollback the run must resume from the last checkpoint before that block,
+    re-split at the next length, and end on GMP's Res64, redoing at most the failing block and
+    the one in flight behind it."""
+    want = {(q, k): r for q, k, r in load_interim_fixture("rollback_gmp.csv")}
+    k = 6000
+    assert llhip.get_checkpoint_blocks() == 1
+    probe = llhip.lucas_lehmer_n(p, k, llhip.FP64, n, no_rollback=True)
+    first = int((llhip.roundoff_trace() > 0.4).argmax())
+    assert probe["status"] == llhip.ST_ROUNDOFF and 200 <= first < k - 200, (probe, first)
+    r = llhip.lucas_lehmer_n(p, k, llhip.FP64, n)
+    st = llhip.last_run_stats()
+    assert r["status"] == llhip.ST_OK and r["n_used"] == n_next, r
+    assert r["res64"] == want[(p, k)]
+    assert st["rollbacks"] == 1 and st["resumed"] == 1 and st["resumed_at"] == (first // 200) * 200, (st, first)
+    assert st["squarings"] <= k + 2 * 200, st
+    assert len(llhip.roundoff_trace()) == k - st["resumed_at"]
+
+
+def test_rollback_without_checkpoints_restarts_and_stays_exact():
+    p, n, n_next = ROLLBACK[0]
+    want = {(q, k): r for q, k, r in load_interim_fixture("rollback_gmp.csv")}
+    try:
+        llhip.set_checkpoint_blocks(0)
+        r = llhip.lucas_lehmer_n(p, 6000, llhip.FP64, n)
+        st = llhip.last_run_stats()
+    finally:
+        llhip.set_checkpoint_blocks(1)
+    assert r["status"] == llhip.ST_OK and r["n_used"] == n_next and r["res64"] == want[(p, 6000)]
+    assert st["rollbacks"] == 1 and st["resumed"] == 0 and st["squarings"] > 6000
