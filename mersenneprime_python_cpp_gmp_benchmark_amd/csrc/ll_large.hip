@@ -205,6 +205,17 @@ LL_DEV void st_coh(i64 *p, i64 v) { st_coh((u64 *)p, (u64)v); }
 LL_DEV void ld_coh(const double *p, double &v) { v = __longlong_as_double((long long)ld_coh_u((const u64 *)p)); }
 LL_DEV void ld_coh(const i64 *p, i64 &v) { v = (i64)ld_coh_u((const u64 *)p); }
 
+// Element stores that the next launch reads: non-temporal, so they stream to memory during the
+// launch instead of sitting dirty in this XCD's L2 until the write-back at the launch's end.
+LL_DEV void st_stream(C64 *p, C64 v) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    d2 t;
+    t.x = v.re;
+    t.y = v.im;
+    __builtin_nontemporal_store(t, reinterpret_cast<d2 *>(p));
+}
+LL_DEV void st_stream(u64 *p, u64 v) { __builtin_nontemporal_store(v, p); }
+
 // wait for this thread's vector-memory operations only (vmcnt = 0; expcnt and lgkmcnt at
 // their "no wait" maxima, gfx9 encoding)
 LL_DEV void wait_vmem() { __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8)); }
@@ -228,14 +239,15 @@ LL_DEV void carry_from(i64 v, i64 &c) { c = v; }
 // that it raises *fault and stops waiting, so the launch always drains (and once *fault is up,
 // later waits give up within a few polls).
 constexpr u64 kSpinTicks = 200000000ull;   // 2 s at the usual 100 MHz wall clock
+LL_DEV bool tag_ok(u64 w, u32 seq) { return (u32)(w >> kTagShift) == (seq & 0xFFFFu); }
 LL_DEV u64 wait_tagged(const u64 *p, u32 seq, unsigned *fault) {
     u64 w = ld_coh_u(p);
-    if ((u32)(w >> kTagShift) == (seq & 0xFFFFu)) return w;
+    if (tag_ok(w, seq)) return w;
     const u64 t0 = wall_clock64();
     for (u32 k = 1;; ++k) {
         __builtin_amdgcn_s_sleep(1);
         w = ld_coh_u(p);
-        if ((u32)(w >> kTagShift) == (seq & 0xFFFFu)) return w;
+        if (tag_ok(w, seq)) return w;
         if (wall_clock64() - t0 > kSpinTicks ||
             ((k & 63) == 0 && __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
             atomicOr(fault, 1u);
@@ -452,7 +464,7 @@ LL_DEV void col_forward(typename Col<IS_GL, W, H>::T *lds_all, TP tw,
     for (int q = 0; q < kR; ++q) {
         if constexpr (IS_GL) x[q] = gl_mul(x[q], tm[q]);
         else x[q] = cmul(x[q], tm[q]);
-        elem[w + W * (t + C::GH * q)] = x[q];
+        st_stream(elem + w + W * (t + C::GH * q), x[q]);
     }
 }
 
@@ -617,12 +629,20 @@ __global__ void __launch_bounds__(kCW * (H / kR)) lg_cols_fused(LargeArgs A) {
     }
     if (wide) atomicOr(A.fault, 2u);            // a carry the tagged word cannot hold: fail the run
     if constexpr (!IS_GL) atomicMax(&s_err, (unsigned long long)__double_as_longlong(eloc));
+    // the first poll of every word is issued before any is waited on: one round trip, not NROW
     CT cin[C::NROW];
+    u64 cw[C::NROW];
+    int src[C::NROW];
 #pragma unroll
     for (int i = 0; i < C::NROW; ++i) {
         const int h = tid + C::NT * i;
-        const int src = blk > 0 ? (blk - 1) * H + h : (NBLK - 1) * H + (h > 0 ? h - 1 : H - 1);
-        carry_from(untag(wait_tagged(tc + src, seq, A.fault)), cin[i]);
+        src[i] = blk > 0 ? (blk - 1) * H + h : (NBLK - 1) * H + (h > 0 ? h - 1 : H - 1);
+        cw[i] = ld_coh_u(tc + src[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < C::NROW; ++i) {
+        if (!tag_ok(cw[i], seq)) cw[i] = wait_tagged(tc + src[i], seq, A.fault);
+        carry_from(untag(cw[i]), cin[i]);
     }
     col_forward<IS_GL, W, H>(lds_all, (const T *)s_tw, wf, tm, A, blk, c, t, tid, cin);
     // the workgroup's round-off maximum (s_err, complete after the barriers of col_forward) goes
@@ -697,7 +717,7 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
         row_transform<F, W, GW, true>(x, lds, t, (const T *)s_tw);
     }
 #pragma unroll
-    for (int s = 0; s < kR; ++s) elem[t + GW * s] = x[s];
+    for (int s = 0; s < kR; ++s) st_stream(elem + t + GW * s, x[s]);
 }
 
 // ---- carry-in only (after the last squaring, before normalisation) ----------------------------
