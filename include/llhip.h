@@ -175,6 +175,19 @@ int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n
  * {1, 3, 5} * 2^k (lucaslehmer.cpp:250-252). */
 size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap);
 
+/* Schedules of the large engine at length n (digit count): the H x W split of its transform
+ * elements and the column-pass block width CW.  Writes up to cap entries into w, h, cw (any may be
+ * null) and returns the count (0: not a large-engine length), or a negative status.  No reference
+ * counterpart: the reference fixes its one schedule (cuda/ll_ntt.cu:224, 256-thread blocks). */
+int llh_large_schedules(int transform, uint64_t n, int32_t *w, int32_t *h, int32_t *cw, int cap);
+
+/* Runs length n at schedule `sched` (an index of llh_large_schedules) from now on; -1 returns it
+ * to the measured choice compiled into the library (the autotuner's table; the list's first
+ * schedule where the table has no entry).  llh_get_large_schedule returns the index in force.
+ * No reference counterpart (see llh_large_schedules). */
+int llh_set_large_schedule(int transform, uint64_t n, int sched);
+int llh_get_large_schedule(int transform, uint64_t n);
+
 /* Per-iteration maximum round-off of the last large-engine FP64 run (one value per squaring);
  * writes up to cap values and returns how many there are.  The reference only keeps the
  * maximum over the run (lucaslehmer.cpp:289-295). */

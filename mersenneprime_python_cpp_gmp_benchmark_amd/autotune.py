@@ -1,4 +1,5 @@
-"""Autotuner: measured cost of one squaring at every engine length.
+"""Autotuner: measured cost of one squaring at every engine length, and the large engine's
+schedule (H x W split and column-pass block width) per length.
 
 Large engine: for each supported length N >= 2^16 the engine runs a short burst of
 squarings of an exponent at 17 bits per digit and records the device time per squaring.
@@ -15,40 +16,169 @@ csrc/ll_batch_costs.inc (the library's defaults) and batch_costs.json.
 writes length_costs.json next to this file;
     python -m mersenneprime_python_cpp_gmp_benchmark_amd.autotune --batch
 writes batch_costs.json and csrc/ll_batch_costs.inc (rebuild to make them the defaults).
+
+Large-engine schedules (--schedules): every schedule of every large length (both transforms)
+runs a burst of squarings alone on the GPU; the fastest becomes the length's schedule
+(csrc/ll_large_sched_defaults.inc, the library's default after a rebuild; large_schedules.json
+keeps every measurement), and the length costs are then measured at the chosen schedules.
+
+Every cost table records a hash of the kernel sources it was measured on (comments and blank
+space ignored); kernel_source_hash() recomputes it, so a test can tell a stale table.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
+import re
 import sys
 
 from . import llhip
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 COSTS_PATH = os.path.join(HERE, "length_costs.json")
+CSRC = os.path.join(HERE, "csrc")
+SCHED_INC = os.path.join(CSRC, "ll_large_sched_defaults.inc")
+SCHED_JSON = os.path.join(HERE, "large_schedules.json")
+
+# the sources whose code decides each table's costs
+KERNEL_SOURCES = {
+    "large": ["ll_large_impl.hpp", "ll_large_scheds.inc", "ll_large_sched_defaults.inc", "ll_dft.hpp",
+              "ll_common.hpp"],
+    "batch": ["ll_batch.hip", "ll_dft.hpp", "ll_common.hpp"],
+}
+
+
+def kernel_source_hash(kind: str) -> str:
+    """sha256 (16 hex digits) of the kernel sources of one table, comments and blank space removed."""
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES[kind]:
+        with open(os.path.join(CSRC, name)) as f:
+            text = f.read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        h.update(name.encode())
+        h.update(re.sub(r"\s+", "", text).encode())
+    return h.hexdigest()[:16]
+
+
+def prime_at_most(hi: int) -> int:
+    """Largest prime <= hi (deterministic Miller-Rabin, exact below 3.3e24)."""
+    def is_prime(n: int) -> bool:
+        if n < 2:
+            return False
+        for q in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            if n % q == 0:
+                return n == q
+        d, r = n - 1, 0
+        while d % 2 == 0:
+            d //= 2
+            r += 1
+        for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            x = pow(a, d, n)
+            if x in (1, n - 1):
+                continue
+            for _ in range(r - 1):
+                x = x * x % n
+                if x == n - 1:
+                    break
+            else:
+                return False
+        return True
+    while not is_prime(hi):
+        hi -= 1
+    return hi
 
 
 LOADED_JOBS = 4     # exponents in flight for the loaded cost (the large engine's streams)
 
 
-def measure(iters: int = 600, min_n: int = 1 << 16) -> tuple[dict, dict]:
+def per_squaring_us(run, iters: int, reps: int = 2) -> float:
+    """Device microseconds per squaring of `run(k)` (device seconds of k squarings): the slope
+    between bursts of iters // 4 and iters squarings, so what a run costs once (set-up, graph
+    capture, normalisation) cancels; best of `reps`."""
+    k0 = max(iters // 4, 1)
+    t0 = min(run(k0) for _ in range(reps))
+    t1 = min(run(iters) for _ in range(reps))
+    return (t1 - t0) / (iters - k0) * 1e6
+
+
+def measure(iters: int = 800, min_n: int = 1 << 16) -> tuple[dict, dict]:
     """(alone, loaded): device microseconds per squaring at every large-engine length, with the
     exponent alone on the GPU and with LOADED_JOBS exponents of that length in one call."""
     alone, loaded = {}, {}
+
+    def one(p, n):
+        return lambda k: llhip.lucas_lehmer_n(p, k, llhip.FP64, n, no_rollback=True)["seconds"]
+
+    def four(p, n):
+        def run(k):
+            llhip.batch_n([p] * LOADED_JOBS, k, llhip.FP64, n, no_rollback=True)
+            return llhip.last_launch_stats()[0] / 1e3
+        return run
+
     for n in llhip.supported_lengths(llhip.FP64):
         if n < min_n:
             continue
-        p = int(17 * n) | 1
+        p = prime_at_most(17 * n)
         llhip.lucas_lehmer_n(p, 50, llhip.FP64, n, no_rollback=True)       # tables + warm-up
-        r = llhip.lucas_lehmer_n(p, iters, llhip.FP64, n, no_rollback=True)
-        alone[str(n)] = r["seconds"] / iters * 1e6
-        llhip.batch_n([p] * LOADED_JOBS, iters, llhip.FP64, n, no_rollback=True)
-        ms, _ = llhip.last_launch_stats()
-        loaded[str(n)] = ms * 1e3 / (LOADED_JOBS * iters)
+        alone[str(n)] = per_squaring_us(one(p, n), iters)
+        loaded[str(n)] = per_squaring_us(four(p, n), iters) / LOADED_JOBS
         print(f"N={n:8d} {alone[str(n)]:8.2f} us/squaring alone, {loaded[str(n)]:8.2f} with {LOADED_JOBS} in flight",
               flush=True)
     return alone, loaded
+
+
+def measure_schedules(iters: int = 800, transforms=(llhip.FP64, llhip.NTT), reps: int = 2) -> dict:
+    """{(transform, N): [(W, H, CW, us per squaring)]}: every schedule of every large length, the
+    exponent (the largest prime <= 16.5 N) alone on the GPU, best of `reps` bursts."""
+    out = {}
+    for tr in transforms:
+        for n in llhip.supported_lengths(tr):
+            sch = llhip.large_schedules(n, tr)
+            if not sch:
+                continue
+            p = prime_at_most(int(16.5 * n))
+            rows = []
+            try:
+                for s, (w, h, cw) in enumerate(sch):
+                    llhip.set_large_schedule(n, s, tr)
+                    llhip.lucas_lehmer_n(p, 50, tr, n, no_rollback=True)       # tables, graphs, warm-up
+                    us = per_squaring_us(lambda k: llhip.lucas_lehmer_n(p, k, tr, n, no_rollback=True)["seconds"],
+                                         iters, reps)
+                    rows.append((w, h, cw, us))
+            finally:
+                llhip.set_large_schedule(n, -1, tr)
+            out[(tr, n)] = rows
+            best = min(rows, key=lambda r: r[3])
+            print(f"{'fp64' if tr == llhip.FP64 else 'ntt '} N={n:8d}: " +
+                  "  ".join(f"{w}x{h}/{cw} {us:6.2f}" for w, h, cw, us in rows) +
+                  f"  -> {best[0]}x{best[1]}/{best[2]}", flush=True)
+    return out
+
+
+def write_schedules(meas: dict, iters: int) -> dict:
+    """Writes the fastest schedule per length (SCHED_INC, SCHED_JSON); returns {(tr, N): index}."""
+    dev = llhip.device_info()
+    choice = {}
+    with open(SCHED_INC, "w") as f:
+        f.write("// Measured schedule of each large-engine length: LLD(transform, N, W, H, CW), transform 0 FP64,\n"
+                "// 1 NTT.  Written by `python -m mersenneprime_python_cpp_gmp_benchmark_amd.autotune --schedules`\n"
+                f"// on {dev['name'] or dev['arch']} ({dev['arch']}): the fastest schedule of ll_large_scheds.inc with\n"
+                f"// one exponent alone on the GPU ({iters} squarings); a length without a row runs the first one.\n")
+        for (tr, n), rows in sorted(meas.items()):
+            i = min(range(len(rows)), key=lambda k: rows[k][3])
+            choice[(tr, n)] = i
+            w, h, cw, us = rows[i]
+            f.write(f"LLD({tr}, {n}, {w}, {h}, {cw})\n")
+    with open(SCHED_JSON, "w") as f:
+        json.dump({"device": dev["name"] or dev["arch"], "arch": dev["arch"], "iters": iters,
+                   "kernel_hash": kernel_source_hash("large"),
+                   "us_per_squaring": {f"{'fp64' if tr == llhip.FP64 else 'ntt'}:{n}":
+                                       {f"{w}x{h}/{cw}": round(us, 3) for w, h, cw, us in rows}
+                                       for (tr, n), rows in sorted(meas.items())}}, f, indent=1)
+    return choice
 
 
 BATCH_JSON = os.path.join(HERE, "batch_costs.json")
@@ -105,6 +235,7 @@ def write_batch(costs: dict, iters: int) -> None:
     dev = llhip.device_info()
     with open(BATCH_JSON, "w") as f:
         json.dump({"device": dev["name"] or dev["arch"], "arch": dev["arch"], "iters": iters,
+                   "kernel_hash": kernel_source_hash("batch"),
                    "us_per_exponent_squaring": {f"{'fp64' if tr == llhip.FP64 else 'ntt'}:{n}": v
                                                 for (tr, n), v in sorted(costs.items())}}, f, indent=1)
     with open(BATCH_INC, "w") as f:
@@ -127,18 +258,25 @@ def load_costs(path: str = COSTS_PATH, loaded: bool = False) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--iters", type=int, default=600)
+    ap.add_argument("--iters", type=int, default=800)
     ap.add_argument("--out", default=COSTS_PATH)
     ap.add_argument("--batch", action="store_true", help="batched-engine lengths")
+    ap.add_argument("--schedules", action="store_true",
+                    help="large engine: pick each length's schedule, then measure the length costs at it")
     a = ap.parse_args()
     if a.batch:
-        it = a.iters if a.iters != 600 else 300
+        it = a.iters if a.iters != 800 else 300
         write_batch(measure_batch(it), it)
         return 0
+    if a.schedules:
+        choice = write_schedules(measure_schedules(), 800)
+        for (tr, n), i in choice.items():           # the costs below run at the chosen schedules
+            llhip.set_large_schedule(n, i, tr)
     alone, loaded = measure(a.iters)
     dev = llhip.device_info()
     with open(a.out, "w") as f:
         json.dump({"device": dev["name"] or dev["arch"], "arch": dev["arch"], "iters": a.iters,
+                   "kernel_hash": kernel_source_hash("large"),
                    "us_per_iter": alone, "loaded_jobs": LOADED_JOBS, "us_per_iter_loaded": loaded}, f, indent=1)
     return 0
 

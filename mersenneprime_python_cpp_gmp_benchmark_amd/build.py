@@ -21,7 +21,8 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("LLH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["ll_batch.hip", "ll_large.hip", "ll_mlp.hip", "ll_capi.hip", "ll_tables.cpp"]
+HIP_SOURCES = ["ll_large_g0.hip", "ll_large_g1.hip", "ll_large_g2.hip", "ll_large_g3.hip", "ll_batch.hip",
+               "ll_large.hip", "ll_mlp.hip", "ll_capi.hip", "ll_tables.cpp"]
 # packed-FP32 instructions are switched off for every device compile
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -37,11 +37,13 @@ int batch_blocks_per_cu(int transform, int family);
 hipError_t batch_resume(int transform, const ResumeReq *d_reqs, int n, i64 *rb_area, i64 *resume, int *d_ok,
                         hipStream_t st);
 
-bool large_launch_for(int transform, u64 M, LargeLaunch &L);
+bool large_launch_for(int transform, u64 M, int sched, LargeLaunch &L);
+std::vector<LargeSched> large_schedules(int transform, u64 M);
 int large_blocks(const LargeLaunch &L);
 hipError_t large_squarings(const LargeLaunch &L, LargeArgs A, int k, hipStream_t st);
 hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
 hipError_t large_normalize(int transform, const void *elem, i64 *dig, const LargeArgs &A, void *out, hipStream_t st);
+size_t large_normalize_bytes(u64 n);
 hipError_t large_relayout(int transform, const i64 *dig, u32 p, u64 n, u64 n2, void *elem2, hipStream_t st);
 void relayout_host(const i64 *dig, u64 p, u64 n, u64 n2, i64 *out);
 
@@ -82,7 +84,10 @@ struct LargeTables {
 
 static std::recursive_mutex g_mx;
 static DeviceState g_dev;
-static std::map<std::pair<int, u64>, LargeTables> g_large_tables;
+static std::map<std::tuple<int, u64, int>, LargeTables> g_large_tables;   // (transform, N, W)
+// Schedule of each large length (index into large_schedules); a length without an entry runs the
+// measured choice compiled in from ll_large_sched_defaults.inc, else the list's first schedule.
+static std::map<std::pair<int, u64>, int> g_large_sched;
 static std::vector<double> g_err_trace;     // per-iteration max round-off of the last large run
 static double g_last_kernel_ms = 0.0;     // kernel time of the last llh_batch call (all its launches)
 static u32 g_last_launches = 0;
@@ -619,16 +624,48 @@ static int ceil_log2(u64 v) {
     return l;
 }
 
-static int get_large_tables(int transform, u64 N, LargeTables &out) {
-    auto key = std::make_pair(transform, N);
+// Measured choice of schedule per large length (autotune.py --schedules writes the file).
+struct LargeSchedDefault {
+    int transform;
+    u64 n;
+    int w, h, cw;
+};
+static const LargeSchedDefault kLargeSchedDefaults[] = {
+#define LLD(TR, NV, WV, HV, CWV) {TR, NV, WV, HV, CWV},
+#include "ll_large_sched_defaults.inc"
+#undef LLD
+    {-1, 0, 0, 0, 0}};
+
+// Schedule index the large engine runs length N at (caller holds g_mx).
+static int large_sched_of(int transform, u64 N) {
+    auto it = g_large_sched.find(std::make_pair(transform, N));
+    if (it != g_large_sched.end()) return it->second;
+    const u64 M = transform == kFP64 ? N / 2 : N;
+    for (const LargeSchedDefault &d : kLargeSchedDefaults) {
+        if (d.transform != transform || d.n != N) continue;
+        const std::vector<LargeSched> v = large_schedules(transform, M);
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i].w == d.w && v[i].h == d.h && v[i].cw == d.cw) return (int)i;
+    }
+    return 0;
+}
+
+static int get_large_tables(int transform, u64 N, int sched, LargeTables &out) {
+    const u64 M0 = transform == kFP64 ? N / 2 : N;
+    const std::vector<LargeSched> sv = large_schedules(transform, M0);
+    if (sched < 0 || sched >= (int)sv.size()) {
+        set_error("no such large-engine schedule for this length");
+        return LLH_ERR_UNSUPPORTED;
+    }
+    auto key = std::make_tuple(transform, N, sv[sched].w);
     auto it = g_large_tables.find(key);
     if (it != g_large_tables.end()) {
         out = it->second;
         return LLH_OK;
     }
-    const u64 M = transform == kFP64 ? N / 2 : N;
+    const u64 M = M0;
     LargeLaunch L;
-    if (!large_launch_for(transform, M, L)) {
+    if (!large_launch_for(transform, M, sched, L)) {
         set_error("no large-engine kernels for this length");
         return LLH_ERR_UNSUPPORTED;
     }
@@ -726,11 +763,12 @@ static int cached_host_malloc(void **p, size_t bytes) {
     return LLH_OK;
 }
 
-// Result of the large engine's device normalisation (lg_normalize's NormOut).
+// Result of the large engine's device normalisation (the NormOut at the start of its buffer).
 struct NormOutH {
     unsigned long long res64;
     int zero;
     int status;
+    int ones;
 };
 
 // One exponent on the large engine at length N, as a resumable job.  Squarings are queued in
@@ -771,7 +809,7 @@ struct LargeJob {
     int ck_next = 0, ck_good = -1;
     NormOutH *noh = nullptr;                 // pinned
     size_t errs_bytes = 0, eh_bytes = 0;
-    size_t bytes_elem = 0, bytes_carry = 0, bytes_dig = 0, bytes_sync = 0;
+    size_t bytes_elem = 0, bytes_carry = 0, bytes_dig = 0, bytes_sync = 0, bytes_nout = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -795,12 +833,13 @@ struct LargeJob {
         r = BatchResult();
         r.status = kStOk;
         const u64 M = transform == kFP64 ? N / 2 : N;
-        if (!N || !large_launch_for(transform, M, L) || (N + 2) * (double)p >= 9.0e15) {
+        const int sched = N ? large_sched_of(transform, N) : 0;
+        if (!N || !large_launch_for(transform, M, sched, L) || (N + 2) * (double)p >= 9.0e15) {
             r.status = kStNotRun;
             return LLH_OK;
         }
         LargeTables T;
-        int rc = get_large_tables(transform, N, T);
+        int rc = get_large_tables(transform, N, sched, T);
         if (rc) return rc;
         const u64 H = (u64)L.h_len;
         const int log_n = (N & (N - 1)) ? 0 : ceil_log2(N);
@@ -813,8 +852,9 @@ struct LargeJob {
         bytes_elem = M * esz;
         bytes_carry = nblk * H * 8;
         bytes_dig = N * 8;
+        bytes_nout = large_normalize_bytes(N);
         if ((rc = cached_malloc(&elem, bytes_elem)) || (rc = cached_malloc(&carry, bytes_carry)) ||
-            (rc = cached_malloc(&dig, bytes_dig)) || (rc = cached_malloc(&nout, sizeof(NormOutH))) ||
+            (rc = cached_malloc(&dig, bytes_dig)) || (rc = cached_malloc(&nout, bytes_nout)) ||
             (rc = cached_malloc(&sync, bytes_sync)) || (rc = cached_host_malloc((void **)&noh, sizeof(NormOutH))))
             return rc;
         for (int e = 0; e < 2; ++e)
@@ -1018,7 +1058,7 @@ struct LargeJob {
         g_bufs.put(elem, bytes_elem, false);
         g_bufs.put(carry, bytes_carry, false);
         g_bufs.put(dig, bytes_dig, false);
-        g_bufs.put(nout, sizeof(NormOutH), false);
+        g_bufs.put(nout, bytes_nout, false);
         g_bufs.put(sync, bytes_sync, false);
         g_bufs.put(noh, sizeof(NormOutH), true);
         ev0 = ev1 = nullptr;
@@ -1038,7 +1078,7 @@ static std::vector<u64> large_lengths(int transform) {
         for (int k = 10; k <= 22; ++k) {
             const u64 N = m << k;
             const u64 M = transform == kFP64 ? N / 2 : N;
-            if (large_launch_for(transform, M, L)) v.push_back(N);
+            if (large_launch_for(transform, M, 0, L)) v.push_back(N);
         }
     std::sort(v.begin(), v.end());
     return v;
@@ -1531,6 +1571,48 @@ size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap) {
     if (out)
         for (size_t i = 0; i < v.size() && i < cap; ++i) out[i] = v[i];
     return v.size();
+}
+
+int llh_large_schedules(int transform, uint64_t n, int32_t *w, int32_t *h, int32_t *cw, int cap) {
+    if (transform != kFP64 && transform != kNTT) {
+        set_error("llh_large_schedules: unknown transform");
+        return LLH_ERR_ARG;
+    }
+    const u64 M = transform == kFP64 ? n / 2 : n;
+    const std::vector<LargeSched> v = large_schedules(transform, M);
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) {
+        if (w) w[i] = v[i].w;
+        if (h) h[i] = v[i].h;
+        if (cw) cw[i] = v[i].cw;
+    }
+    return (int)v.size();
+}
+
+int llh_set_large_schedule(int transform, uint64_t n, int sched) {
+    if (transform != kFP64 && transform != kNTT) {
+        set_error("llh_set_large_schedule: unknown transform");
+        return LLH_ERR_ARG;
+    }
+    const u64 M = transform == kFP64 ? n / 2 : n;
+    const int count = (int)large_schedules(transform, M).size();
+    if (sched < -1 || sched >= count) {
+        set_error("llh_set_large_schedule: no such schedule for this length");
+        return LLH_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    if (sched < 0) g_large_sched.erase(std::make_pair(transform, (u64)n));
+    else g_large_sched[std::make_pair(transform, (u64)n)] = sched;
+    return LLH_OK;
+}
+
+int llh_get_large_schedule(int transform, uint64_t n) {
+    const u64 M = transform == kFP64 ? n / 2 : n;
+    if ((transform != kFP64 && transform != kNTT) || large_schedules(transform, M).empty()) {
+        set_error("llh_get_large_schedule: not a large-engine length");
+        return LLH_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(g_mx);
+    return large_sched_of(transform, n);
 }
 
 size_t llh_last_roundoff_trace(double *out, size_t cap) {

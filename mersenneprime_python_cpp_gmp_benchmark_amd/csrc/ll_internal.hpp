@@ -128,7 +128,14 @@ struct LargeLaunch {
     void (*inv)(LargeArgs);
     void (*fused)(LargeArgs);
     void (*apply)(LargeArgs);
-    int w_len, h_len;   // W and H (either may carry the odd factor)
+    int w_len, h_len;   // W and H (W carries the odd factor)
+    int cw;             // columns per column-pass workgroup
+};
+
+// One schedule of the large engine: the H x W split of the M elements and the column-pass
+// block width (ll_large_scheds.inc).
+struct LargeSched {
+    int w, h, cw;
 };
 
 // ---- host helpers (ll_tables.cpp) ---------------------------------------------------------

@@ -76,6 +76,9 @@ def lib():
         L.llh_last_run_stats.argtypes = [P(u64), P(u32), P(u32), P(u64)]
         L.llh_set_checkpoint_blocks.argtypes = [ctypes.c_int]
         L.llh_relayout_host.argtypes = [u64, u64, P(ctypes.c_int64), u64, P(ctypes.c_int64)]
+        L.llh_large_schedules.argtypes = [ctypes.c_int, u64, P(i32), P(i32), P(i32), ctypes.c_int]
+        L.llh_set_large_schedule.argtypes = [ctypes.c_int, u64, ctypes.c_int]
+        L.llh_get_large_schedule.argtypes = [ctypes.c_int, u64]
         _lib = L
     return _lib
 
@@ -284,3 +287,27 @@ def relayout_host(p: int, digits, n_to: int) -> np.ndarray:
     _check(lib().llh_relayout_host(p, len(src), src.ctypes.data_as(P(ctypes.c_int64)), n_to,
                                    out.ctypes.data_as(P(ctypes.c_int64))))
     return out
+
+
+def large_schedules(n: int, transform: int = FP64) -> list[tuple[int, int, int]]:
+    """(W, H, CW) of every large-engine schedule of length n (index = list position)."""
+    L = lib()
+    cnt = L.llh_large_schedules(transform, n, None, None, None, 0)
+    if cnt < 0:
+        _check(cnt)
+    w, h, cw = (np.zeros(max(cnt, 1), np.int32) for _ in range(3))
+    P = ctypes.POINTER(ctypes.c_int32)
+    L.llh_large_schedules(transform, n, w.ctypes.data_as(P), h.ctypes.data_as(P), cw.ctypes.data_as(P), cnt)
+    return [(int(w[i]), int(h[i]), int(cw[i])) for i in range(cnt)]
+
+
+def set_large_schedule(n: int, sched: int, transform: int = FP64) -> None:
+    """Run length n at schedule index `sched` from now on (-1: the library's measured choice)."""
+    _check(lib().llh_set_large_schedule(transform, n, sched))
+
+
+def get_large_schedule(n: int, transform: int = FP64) -> int:
+    r = lib().llh_get_large_schedule(transform, n)
+    if r < 0:
+        _check(r)
+    return r

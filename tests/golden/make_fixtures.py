@@ -14,8 +14,11 @@ the low 64 bits of the canonical residue.  Each fixture is plain CSV:
   above_batch_gmp.csv    the four primes just above 21 x 8192 (beyond the batched FP64
                          lengths, so a batch call runs them on the large engine), full test:
                          p,prime,res64
+  sched_gmp.csv          one exponent per large-engine length N (the largest prime <= 16.5 N),
+                         p,iters,res64 at 201 squarings (two graph blocks: the block boundary's
+                         carry buffer is crossed once)
 
-Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp|above|rollback [--threads T]
+Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp|above|rollback|sched [--threads T]
 """
 from __future__ import annotations
 
@@ -156,9 +159,36 @@ def above_batch(path: str, threads: int) -> None:
     print(f"{len(ps)} exponents in {time.time() - t0:.0f} s")
 
 
+# Large-engine lengths N (digit counts) of both transforms: FP64 N = 2M for M in
+# {1, 3, 5, 7, 15} * 2^k, 2^15 <= M <= 2^21 (M >= 5 * 2^13 for the odd factors); NTT N = M = 2^15 .. 2^21.
+SCHED_ITERS = 201
+
+
+def sched_lengths() -> list:
+    fp64 = [m << k for m in (1, 3, 5, 7, 15) for k in range(13, 22) if (1 << 15) <= (m << k) <= (1 << 21)
+            and (m == 1 or (m << k) >= (5 << 13))]
+    return sorted(set(2 * m for m in fp64) | set(1 << k for k in range(15, 22)))
+
+
+def sched_exponent(n: int) -> int:
+    hi = int(16.5 * n)
+    return O.primes(hi - 20000, hi)[-1]
+
+
+def sched(path: str, threads: int) -> None:
+    ps = sorted(set(sched_exponent(n) for n in sched_lengths()), reverse=True)
+    t0 = time.time()
+    out = O.ll_many(ps, iters=SCHED_ITERS, engine=0, threads=threads)
+    with open(path, "w") as f:
+        f.write("p,iters,res64\n")
+        for p, (_, r) in sorted(zip(ps, out)):
+            f.write(f"{p},{SCHED_ITERS},{r:016x}\n")
+    print(f"{len(ps)} schedule-test residues in {time.time() - t0:.0f} s")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above", "rollback"])
+    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above", "rollback", "sched"])
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--order", choices=["asc", "desc"], default="asc")
@@ -174,6 +204,8 @@ def main() -> None:
         above_batch(os.path.join(HERE, "above_batch_gmp.csv"), a.threads)
     elif a.what == "rollback":
         rollback(os.path.join(HERE, "rollback_gmp.csv"), a.threads)
+    elif a.what == "sched":
+        sched(os.path.join(HERE, "sched_gmp.csv"), a.threads)
     elif a.what == "c4list":
         c4_list(os.path.join(HERE, "c4_exponents.csv"))
     else:

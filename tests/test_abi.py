@@ -69,3 +69,30 @@ def test_cuda_module_length_rule_equals_reference_rule():
     for n, top in ((64, 1792), (128, 3584), (256, 6912), (512, 13824), (1024, 26624), (2048, 53248),
                    (4096, 102400), (8192, 204800)):
         assert lc.ntt_length(top) == n and lc.ntt_length(top + 1) == (2 * n if n < 8192 else 0)
+
+
+def test_large_schedule_selection_is_host_only_and_checked():
+    """The schedule list of a large length, its selection and its limits (no device call)."""
+    for n, tr in ((1 << 16, llhip.FP64), (1 << 20, llhip.FP64), (917504, llhip.FP64), (1 << 20, llhip.NTT)):
+        sch = llhip.large_schedules(n, tr)
+        m = n // 2 if tr == llhip.FP64 else n
+        assert sch and all(w * h == m for w, h, _ in sch), (n, sch)
+        # a column-pass block reads whole 128-byte runs in at least one schedule
+        esz = 16 if tr == llhip.FP64 else 8
+        assert any(cw * esz == 128 for _, _, cw in sch), (n, sch)
+        cur = llhip.get_large_schedule(n, tr)
+        try:
+            for s in range(len(sch)):
+                llhip.set_large_schedule(n, s, tr)
+                assert llhip.get_large_schedule(n, tr) == s
+        finally:
+            llhip.set_large_schedule(n, -1, tr)
+        assert llhip.get_large_schedule(n, tr) == cur
+    assert llhip.large_schedules(4096, llhip.FP64) == []          # a batched length
+    assert llhip.large_schedules(917504, llhip.NTT) == []         # no 7 * 2^k NTT
+    for bad in ((1 << 20, 99, llhip.FP64), (4096, 0, llhip.FP64), (1 << 20, -2, llhip.NTT)):
+        try:
+            llhip.set_large_schedule(*bad)
+        except RuntimeError:
+            continue
+        raise AssertionError(f"accepted {bad}")
