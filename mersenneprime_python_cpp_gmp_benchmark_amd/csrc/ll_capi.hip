@@ -44,7 +44,8 @@ hipError_t large_squarings(const LargeLaunch &L, LargeArgs A, int k, hipStream_t
 hipError_t large_apply_carry(const LargeLaunch &L, const LargeArgs &A, hipStream_t st);
 hipError_t large_normalize(int transform, const void *elem, i64 *dig, const LargeArgs &A, void *out, hipStream_t st);
 size_t large_normalize_bytes(u64 n);
-hipError_t large_relayout(int transform, const i64 *dig, u32 p, u64 n, u64 n2, void *elem2, hipStream_t st);
+hipError_t large_relayout(int transform, const i64 *dig, u32 p, u64 n, u64 n2, void *elem2, u32 w2, u32 h2,
+                          hipStream_t st);
 void relayout_host(const i64 *dig, u64 p, u64 n, u64 n2, i64 *out);
 
 static thread_local std::string g_err;
@@ -847,7 +848,7 @@ struct LargeJob {
         const u64 nblk = (u64)large_blocks(L);
         errs_bytes = kGraphBlock * 8;
         eh_bytes = (kGraphBlock + 1) * 8;
-        // sync: the fused pass's launch counter, the hand-off fault word, its tagged carry words
+        // sync: 8 unused bytes, the hand-off fault word, the fused pass's tagged carry words
         bytes_sync = 16 + nblk * H * 8;
         bytes_elem = M * esz;
         bytes_carry = nblk * H * 8;
@@ -867,7 +868,8 @@ struct LargeJob {
         HIPCHK(hipMemsetAsync(carry, 0, bytes_carry, st));
         HIPCHK(hipMemsetAsync(sync, 0, bytes_sync, st));
         if (from) {
-            HIPCHK(large_relayout(transform, (const i64 *)from->dig, (u32)p, from->n, N, elem, st));
+            HIPCHK(large_relayout(transform, (const i64 *)from->dig, (u32)p, from->n, N, elem, (u32)L.w_len,
+                                  (u32)L.h_len, st));
             done = queued = from->done;
         } else {
             HIPCHK(hipMemsetAsync(elem, 0, M * esz, st));
@@ -886,10 +888,12 @@ struct LargeJob {
         A.p = (u32)p; A.log_n = (u32)log_n; A.log_s = T.log_s; A.log_sw = T.log_sw; A.err_slot = 0;
         A.n = N; A.inv_n = 1.0 / (double)N;
         A.p_mod_n = (u32)(p % N); A.p_div_n = (u32)(p / N);
-        A.ticket = (unsigned long long *)sync;
+        A.tag = 1;
         A.fault = (unsigned *)((char *)sync + 8);
         A.tcarry = (char *)sync + 16;
         A.xcd_blocks = alone ? 1u : 0u;
+        A.w_len = (u32)L.w_len;
+        A.h_len = (u32)L.h_len;
         trace.clear();
         trace.reserve(iters - done);
         for (int e = 0; e < 2; ++e) HIPCHK(hipEventCreateWithFlags(&evb[e], hipEventDisableTiming));
@@ -908,6 +912,10 @@ struct LargeJob {
         const int e = (int)(nblocks % 2);
         LargeArgs Ab = A;
         Ab.err = (unsigned long long *)errs[e];
+        // carry-word tags 1 + 256 e + i: consecutive fused launches differ, within a block by i and
+        // across blocks by the parity e (only the last block is short, so no two short blocks meet),
+        // and no tag is 0, the cleared words' tag
+        Ab.tag = 1u + 256u * (u32)e;
         HIPCHK(hipMemsetAsync(errs[e], 0, errs_bytes, st));
         if (k == (u64)kGraphBlock) {
             if (!gexec[e]) {

@@ -116,10 +116,11 @@ struct LargeArgs {
     u64 n;                            // digit count N
     double inv_n;                     // 1.0 / N
     u32 p_mod_n, p_div_n;             // p mod N, floor(p / N)
-    unsigned long long *ticket;       // fused column pass: launch counter (only grows)
+    u32 tag;                          // fused column pass: this launch's carry-word tag (16 bits, nonzero)
     void *tcarry;                     // fused column pass: [nblk][H] tagged carry words
     unsigned *fault;                  // raised when a carry hand-off fails (timeout, carry range)
     u32 xcd_blocks;                   // fused column pass: XCD-aware block order (job runs alone)
+    u32 w_len, h_len;                 // W and H of the schedule (the element layout's shape)
 };
 
 struct LargeLaunch {

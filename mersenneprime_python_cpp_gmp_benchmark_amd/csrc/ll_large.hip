@@ -45,16 +45,26 @@ LL_DEV i64 add_run(i64 *dig, u64 a, u64 b, i64 c, u32 p, const NDiv &D) {
     return c;
 }
 
+// Storage index of digit d's word: FP64 digit d is part d % 2 of element d / 2 (a word of two
+// doubles), GL digit d is element d; the elements are stored by elem_index of the W x H layout.
 template <bool IS_GL>
-__global__ void __launch_bounds__(256) lg_norm_local(const void *elem_v, i64 *dig, i64 *segc, u32 p, NDiv D) {
+LL_DEV u64 digit_word(u64 d, u32 W, u32 H) {
+    const u64 j = IS_GL ? d : d / 2;
+    const u64 e = elem_index((u32)(j / W), (u32)(j % W), W, H, ElemOf<IS_GL>::LC);
+    return IS_GL ? e : 2 * e + (d & 1);
+}
+
+template <bool IS_GL>
+__global__ void __launch_bounds__(256) lg_norm_local(const void *elem_v, i64 *dig, i64 *segc, u32 p, NDiv D,
+                                                     u32 W, u32 H) {
     const u64 s = (u64)blockIdx.x * 256u + threadIdx.x;
     if (s >= D.n / kNormK) return;
     const u64 d0 = s * kNormK;
     i64 v[kNormK];
 #pragma unroll
     for (int k = 0; k < kNormK; ++k) {
-        if constexpr (IS_GL) v[k] = (i64)reinterpret_cast<const u64 *>(elem_v)[d0 + k];
-        else v[k] = (i64)reinterpret_cast<const double *>(elem_v)[d0 + k];
+        if constexpr (IS_GL) v[k] = (i64)reinterpret_cast<const u64 *>(elem_v)[digit_word<true>(d0 + k, W, H)];
+        else v[k] = (i64)reinterpret_cast<const double *>(elem_v)[digit_word<false>(d0 + k, W, H)];
     }
     i64 c = 0;
     u64 lo = cbit(d0, p, D);
@@ -140,22 +150,26 @@ namespace {
 // One thread per output digit; the output is the element array of the N'-digit engine (FP64:
 // N' doubles, element j = digits 2j, 2j + 1; GL: N' words holding the signed digit).
 template <bool IS_GL>
-__global__ void __launch_bounds__(256) lg_relayout(const i64 *dig, u32 p, u64 n, u64 n2, void *elem2) {
+__global__ void __launch_bounds__(256) lg_relayout(const i64 *dig, u32 p, u64 n, u64 n2, void *elem2, u32 W2,
+                                                   u32 H2) {
     const u64 k = (u64)blockIdx.x * 256ull + threadIdx.x;
     if (k >= n2) return;
     const i64 v = relayout_digit(dig, p, n, n2, k);
-    if constexpr (IS_GL) reinterpret_cast<u64 *>(elem2)[k] = (u64)v;
-    else reinterpret_cast<double *>(elem2)[k] = (double)v;
+    if constexpr (IS_GL) reinterpret_cast<u64 *>(elem2)[digit_word<true>(k, W2, H2)] = (u64)v;
+    else reinterpret_cast<double *>(elem2)[digit_word<false>(k, W2, H2)] = (double)v;
 }
 
 }  // namespace
 
 namespace llh {
 
-hipError_t large_relayout(int transform, const i64 *dig, u32 p, u64 n, u64 n2, void *elem2, hipStream_t st) {
+// elem2: the element array of the N'-digit job, whose schedule has a W2 x H2 element matrix
+hipError_t large_relayout(int transform, const i64 *dig, u32 p, u64 n, u64 n2, void *elem2, u32 w2, u32 h2,
+                          hipStream_t st) {
     const dim3 grid((unsigned)((n2 + 255) / 256));
-    if (transform == kNTT) hipLaunchKernelGGL(lg_relayout<true>, grid, dim3(256), 0, st, dig, p, n, n2, elem2);
-    else hipLaunchKernelGGL(lg_relayout<false>, grid, dim3(256), 0, st, dig, p, n, n2, elem2);
+    if (transform == kNTT)
+        hipLaunchKernelGGL(lg_relayout<true>, grid, dim3(256), 0, st, dig, p, n, n2, elem2, w2, h2);
+    else hipLaunchKernelGGL(lg_relayout<false>, grid, dim3(256), 0, st, dig, p, n, n2, elem2, w2, h2);
     return hipGetLastError();
 }
 
@@ -215,14 +229,16 @@ int large_blocks(const LargeLaunch &L) { return L.w_len / L.cw; }
 
 // k squarings: forward column pass, then per squaring the row pass and the fused column pass
 // (the last squaring ends with the inverse column pass, leaving its carries in the carry buffer).
-// Squaring i reports its round-off in err slot i.
+// Squaring i reports its round-off in err slot i; its fused pass tags its carry words A.tag + i.
 hipError_t large_squarings(const LargeLaunch &L, LargeArgs A, int k, hipStream_t st) {
     const int W = L.w_len, H = L.h_len;
     const dim3 gcol(W / L.cw), bcol(L.cw * (H / kR));
     const dim3 grow(H / 2), brow(2 * (W / kR));
+    const u32 tag0 = A.tag;
     hipLaunchKernelGGL(L.fwd, gcol, bcol, 0, st, A);
     for (int i = 0; i < k; ++i) {
         A.err_slot = (u32)i;
+        A.tag = tag0 + (u32)i;
         hipLaunchKernelGGL(L.rows, grow, brow, 0, st, A);
         if (i + 1 < k) hipLaunchKernelGGL(L.fused, gcol, bcol, 0, st, A);
         else hipLaunchKernelGGL(L.inv, gcol, bcol, 0, st, A);
@@ -245,8 +261,9 @@ hipError_t large_normalize(int transform, const void *elem, i64 *dig, const Larg
     NormOut *no = (NormOut *)out;
     i64 *segc = (i64 *)((char *)out + kNormSegOffset);
     const dim3 g((unsigned)((A.n / kNormK + 255) / 256));
-    if (transform == kNTT) hipLaunchKernelGGL(lg_norm_local<true>, g, dim3(256), 0, st, elem, dig, segc, A.p, D);
-    else hipLaunchKernelGGL(lg_norm_local<false>, g, dim3(256), 0, st, elem, dig, segc, A.p, D);
+    if (transform == kNTT)
+        hipLaunchKernelGGL(lg_norm_local<true>, g, dim3(256), 0, st, elem, dig, segc, A.p, D, A.w_len, A.h_len);
+    else hipLaunchKernelGGL(lg_norm_local<false>, g, dim3(256), 0, st, elem, dig, segc, A.p, D, A.w_len, A.h_len);
     hipLaunchKernelGGL(lg_norm_carry, dim3(1), dim3(kNormThreads), 0, st, dig, (const i64 *)segc, A.p, D, no);
     hipLaunchKernelGGL(lg_norm_check, g, dim3(256), 0, st, (const i64 *)dig, A.p, D, no);
     hipLaunchKernelGGL(lg_norm_final, dim3(1), dim3(1), 0, st, (const i64 *)dig, A.p, D, no);

@@ -5,8 +5,12 @@
 //
 // Counterpart of the FP64 loop lucaslehmer.cpp:281-311 (and of the NTT loop of
 // cuda/ll_ntt.cu:91-163, which the reference only runs up to N = 8192) for transform lengths
-// 2^15 .. 2^21 elements.  Layout: the M transform elements form an H x W matrix, element
-// j = w + W h (row h, column w; rows contiguous).  With k = k_h + H k_w:
+// 2^15 .. 2^21 elements.  The M transform elements form an H x W matrix, element j = w + W h
+// (row h, column w), stored pair-interleaved in HBM (elem_index): a 128-byte line holds the same
+// LC columns of rows h and H - h (LC = 4 FP64 complex or 8 GL words; row 0 pairs with row H/2).
+// The row pass then reads its row pair as whole lines, and a column-pass block of LC columns
+// reads whole lines too; in plain row-major order two blocks on different XCDs each read and
+// wrote half of every line, which took twice the time for the same bytes.  With k = k_h + H k_w:
 //
 //   Z[k] = sum_w w_W^{w k_w} w_M^{w k_h} sum_h z[w + W h] w_H^{h k_h}
 //
@@ -105,7 +109,17 @@ struct ElemOf {
     typedef typename F::T T;
     typedef typename std::conditional<IS_GL, u64, double>::type WT;   // weight table type
     typedef typename std::conditional<IS_GL, i64, double>::type CT;   // carry type
+    static constexpr u32 LC = IS_GL ? 8 : 4;                           // columns per half line
 };
+
+// Storage index of element (h, w) of the H x W matrix (H even; W a multiple of LC): the line of
+// row pair P = (h < H/2 ? h : H - h) mod H/2 holds LC columns of row P (slot 0) and of row
+// H - P (slot 1; for P = 0, row H/2).
+LL_HD u64 elem_index(u32 h, u32 w, u32 W, u32 H, u32 LC) {
+    const u32 hh = H / 2;
+    const u32 P = (h < hh ? h : H - h) & (hh - 1);
+    return (u64)P * 2 * W + (u64)(w / LC) * (2 * LC) + (h < hh ? 0 : LC) + w % LC;
+}
 
 }  // namespace
 
@@ -222,10 +236,11 @@ LL_DEV void st_stream(u64 *p, u64 v) { __builtin_nontemporal_store(v, p); }
 // their "no wait" maxima, gfx9 encoding)
 LL_DEV void wait_vmem() { __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8)); }
 
-// Tagged carry words of the fused column pass: bits 63..48 hold the launch's sequence number
-// (mod 2^16), bits 47..0 the signed carry.  Every launch rewrites every word, so a word holds
-// either this launch's value or the one before's, whose tag differs; a reader polls its own
-// words until the tag matches, with no flag and no barrier between publishing and reading.
+// Tagged carry words of the fused column pass: bits 63..48 hold the launch's tag (LargeArgs::tag),
+// bits 47..0 the signed carry.  Every launch rewrites every word, so a word holds either this
+// launch's value or the one before's; the host gives consecutive launches different nonzero tags
+// (the words start cleared), so a reader polls its own words until the tag matches, with no flag
+// and no barrier between publishing and reading.
 constexpr u64 kTagShift = 48;
 constexpr i64 kCarryLim = (i64)1 << 47;
 LL_DEV u64 tag_word(u32 seq, i64 c) {
@@ -473,7 +488,7 @@ LL_DEV void col_forward(typename Col<IS_GL, W, H, CW>::T *lds_all, TP tw,
     for (int q = 0; q < kR; ++q) {
         if constexpr (IS_GL) x[q] = gl_mul(x[q], tm[q]);
         else x[q] = cmul(x[q], tm[q]);
-        st_stream(elem + w + W * (t + C::GH * q), x[q]);
+        st_stream(elem + elem_index(t + C::GH * q, w, W, H, C::E::LC), x[q]);
     }
 }
 
@@ -485,7 +500,7 @@ LL_DEV void col_load_inv(typename Col<IS_GL, W, H, CW>::T (&x)[kR], const typena
     const typename C::T *elem = reinterpret_cast<const typename C::T *>(A.elem);
 #pragma unroll
     for (int s = 0; s < kR; ++s) {
-        x[s] = elem[w + W * (t + C::GH * s)];
+        x[s] = elem[elem_index(t + C::GH * s, w, W, H, C::E::LC)];
         if constexpr (IS_GL) x[s] = gl_mul(x[s], tmi[s]);
         else x[s] = cmulc(x[s], tmi[s]);
     }
@@ -512,7 +527,7 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     const CT *carry = reinterpret_cast<const CT *>(A.carry);
     T x[kR];
 #pragma unroll
-    for (int s = 0; s < kR; ++s) x[s] = elem[w + W * (t + C::GH * s)];
+    for (int s = 0; s < kR; ++s) x[s] = elem[elem_index(t + C::GH * s, w, W, H, C::E::LC)];
     CT cin[C::NROW];
 #pragma unroll
     for (int i = 0; i < C::NROW; ++i) {
@@ -564,7 +579,7 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_inv(LargeArgs A) {
         carry[blk * H + h] = cy[i];
         const T *row = lds_all + h * CW;
 #pragma unroll
-        for (int cc = 0; cc < CW; ++cc) elem[blk * CW + cc + (size_t)W * h] = row[cc];
+        for (int cc = 0; cc < CW; ++cc) elem[elem_index(h, blk * CW + cc, W, H, C::E::LC)] = row[cc];
     }
     if constexpr (!IS_GL) err_publish(&s_err, eloc, A, tid);
 }
@@ -579,9 +594,8 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_inv(LargeArgs A) {
 // XCD-aware order (below) the predecessor of the workgroup at blockIdx x < 8 is dispatched near
 // the end; the launch still drains as long as 9 workgroups fit on the GPU at once, because
 // every workgroup that waits has published and the predecessors it waits on only wait on
-// published blocks.  The tag is the launch's sequence number: every workgroup adds one to a
-// counter that only grows, so ticket / NBLK + 1 is the same for all workgroups of one launch
-// and one more than the launch before's.
+// published blocks.  The launch's tag comes with its arguments, so nothing is waited on before
+// the loads go out (a launch counter read with an atomic cost a device-scope round trip first).
 // Column block of workgroup b in the fused pass.  Workgroups are dealt to the 8 XCDs in turn
 // (b mod 8); mapping runs of NBLK / 8 consecutive blocks to one XCD keeps each block's
 // predecessor, whose carries and flag it waits for, on the same XCD for all but 8 hand-offs,
@@ -605,15 +619,12 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fused(LargeArgs A) {
     __shared__ __attribute__((aligned(16))) T s_tw[H];
     __shared__ __attribute__((aligned(16))) T s_twi[IS_GL ? H : 1];
     __shared__ unsigned long long s_err;
-    __shared__ u32 s_seq;
     const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
     const int blk = A.xcd_blocks ? xcd_block<NBLK>(blockIdx.x) : (int)blockIdx.x, w = blk * CW + c;
-    if (tid == 0) {
-        s_seq = (u32)(atomicAdd(A.ticket, 1ull) / (unsigned long long)NBLK) + 1u;
-        s_err = 0;
-    }
-    stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
-    if constexpr (IS_GL) stage_col_table<T, H, C::NT>(s_twi, A.twHi, tid);
+    if (tid == 0) s_err = 0;
+    // Every global load of the thread goes out before the first wait: the element, twiddle and
+    // weight loads, then the twiddle table staged into LDS, whose stores wait for all of them
+    // (staged first, the table's round trip came before the others were issued).
     T tm[kR];
     col_twiddles<IS_GL, W, H, CW>(tm, gtab<T>(A.tmL, A.tmH, A.log_s), w, t);
     T x[kR];
@@ -627,10 +638,12 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fused(LargeArgs A) {
     WT wi[C::NWT], wf[C::NWT];
     col_weights<IS_GL, W, H, CW>(wi, gtab<WT>(A.wiL, A.wiH, A.log_sw), A, w, t);
     col_weights<IS_GL, W, H, CW>(wf, gtab<WT>(A.wfL, A.wfH, A.log_sw), A, w, t);
+    stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
+    if constexpr (IS_GL) stage_col_table<T, H, C::NT>(s_twi, A.twHi, tid);
     CT cy[C::NROW];
     double eloc = 0.0;
     col_inverse<IS_GL, W, H, CW>(x, lds_all, (const T *)(IS_GL ? s_twi : s_tw), wi, A, blk, c, t, tid, cy, eloc);
-    const u32 seq = s_seq;   // written before the barriers inside col_inverse
+    const u32 seq = A.tag;
     u64 *tc = reinterpret_cast<u64 *>(A.tcarry);
     bool wide = false;
 #pragma unroll
@@ -690,11 +703,14 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
     const int i = blockIdx.x;
     const int row = g == 0 ? i : (i == 0 ? H / 2 : H - i);
     T *lds = lds_all + g * REG;
-    T *elem = reinterpret_cast<T *>(A.elem) + (size_t)W * row;
+    // rows i and H - i (i = 0: rows 0 and H/2) are slots 0 and 1 of row pair i's lines
+    T *elem = reinterpret_cast<T *>(A.elem) + (size_t)i * 2 * W + g * E::LC;
+    static_assert(GW % E::LC == 0, "a thread's elements must start a half line alike");
+    const int tl = (t / (int)E::LC) * 2 * (int)E::LC + t % (int)E::LC;   // element t of the row
 
     T x[kR];
 #pragma unroll
-    for (int s = 0; s < kR; ++s) x[s] = elem[t + GW * s];
+    for (int s = 0; s < kR; ++s) x[s] = elem[tl + 2 * GW * s];
     C64 wk[IS_GL ? 1 : kR];
     if constexpr (!IS_GL) {
 #pragma unroll
@@ -730,7 +746,7 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
         row_transform<F, W, GW, true>(x, lds, t, (const T *)s_tw);
     }
 #pragma unroll
-    for (int s = 0; s < kR; ++s) st_stream(elem + t + GW * s, x[s]);
+    for (int s = 0; s < kR; ++s) st_stream(elem + tl + 2 * GW * s, x[s]);
 }
 
 // ---- carry-in only (after the last squaring, before normalisation) ----------------------------
@@ -749,9 +765,9 @@ __global__ void __launch_bounds__(256) lg_apply_carry(LargeArgs A) {
         CT cin;
         if (blk > 0) cin = carry[(blk - 1) * H + h];
         else cin = h > 0 ? carry[(NBLK - 1) * H + h - 1] : carry[(NBLK - 1) * H + H - 1];
-        T *row = elem + blk * CW + (size_t)W * h;
         const u64 j0 = (u64)blk * CW + (u64)W * h;
         for (int cc = 0; cc < CW; ++cc) {
+            T *row = elem + elem_index(h, blk * CW + cc, W, H, E::LC) - cc;   // row[cc]: element (h, blk CW + cc)
             if constexpr (IS_GL) {
                 i64 v = (i64)row[cc] + cin;
                 if (cc == CW - 1) { row[cc] = (u64)v; break; }
