@@ -114,21 +114,28 @@ __global__ void __launch_bounds__(kNormThreads) lg_norm_carry(i64 *dig, const i6
     }
 }
 
+// The flags are combined over the workgroup first, so each workgroup sends at most two atomics
+// (one per thread kept 32 K of them queued on the same two words at N = 2^20).
 __global__ void __launch_bounds__(256) lg_norm_check(const i64 *dig, u32 p, NDiv D, NormOut *out) {
     const u64 s = (u64)blockIdx.x * 256u + threadIdx.x;
-    if (s >= D.n / kNormK) return;
-    const u64 d0 = s * kNormK;
     bool zero = true, ones = true;
-    u64 lo = cbit(d0, p, D);
-    for (int k = 0; k < kNormK; ++k) {
-        const u64 hi = cbit(d0 + k + 1, p, D);
-        const i64 v = dig[d0 + k];
-        zero = zero && v == 0;
-        ones = ones && v == digit_mask((int)(hi - lo));
-        lo = hi;
+    if (s < D.n / kNormK) {
+        const u64 d0 = s * kNormK;
+        u64 lo = cbit(d0, p, D);
+        for (int k = 0; k < kNormK; ++k) {
+            const u64 hi = cbit(d0 + k + 1, p, D);
+            const i64 v = dig[d0 + k];
+            zero = zero && v == 0;
+            ones = ones && v == digit_mask((int)(hi - lo));
+            lo = hi;
+        }
     }
-    if (!zero) atomicAnd(&out->zero, 0);
-    if (!ones) atomicAnd(&out->ones, 0);
+    const int all_zero = __syncthreads_and(zero ? 1 : 0);
+    const int all_ones = __syncthreads_and(ones ? 1 : 0);
+    if (threadIdx.x == 0) {
+        if (!all_zero) atomicAnd(&out->zero, 0);
+        if (!all_ones) atomicAnd(&out->ones, 0);
+    }
 }
 
 __global__ void lg_norm_final(const i64 *dig, u32 p, NDiv D, NormOut *out) {
