@@ -66,3 +66,35 @@ def test_learned_length_is_exact_and_not_longer():
     held = exponents("heldout")
     shorter = sum(nselect.choose_length(p, model, loaded=True) < nselect.conservative_length(p) for p in held)
     assert shorter >= len(held) // 8
+
+
+@pytest.mark.skipif(not os.path.exists(nselect.MODEL_PATH), reason="no trained round-off model committed")
+def test_learned_lengths_save_device_time_on_heldout():
+    """What the trained network gains over the method without it (the reference's conservative
+    rule, lucaslehmer.cpp:244-254): the 64 held-out exponents in one call, 2,000 squarings each,
+    at the conservative lengths and at the learned ones (costs priced with several exponents in
+    flight), twice each, alternating.  The learned call is faster on its best time, rolls nothing
+    back, stays within the round-off limit, and gives the same Res64 as the conservative call."""
+    from mersenneprime_python_cpp_gmp_benchmark_amd.roundoff_data import exponents
+    ps = exponents("heldout")
+    model = nselect.load_model()
+    nc = [nselect.conservative_length(p) for p in ps]
+    nl = [nselect.choose_length(p, model, loaded=True) for p in ps]
+    k = 2000
+    times = {"conservative": [], "learned": []}
+    outs = {}
+    for _ in range(2):
+        for name, ns in (("conservative", nc), ("learned", nl)):
+            outs[name] = llhip.batch_n(ps, k, llhip.FP64, ns)
+            times[name].append(llhip.last_launch_stats()[0])
+    cons, lrn = outs["conservative"], outs["learned"]
+    assert (cons["status"] == llhip.ST_OK).all() and (lrn["status"] == llhip.ST_OK).all()
+    assert [int(n) for n in lrn["n_used"]] == nl            # no rollback at a learned length
+    assert float(np.max(lrn["maxerr"])) < 0.4
+    assert [int(r) for r in lrn["res64"]] == [int(r) for r in cons["res64"]]
+    shorter = sum(a < b for a, b in zip(nl, nc))
+    gain = min(times["conservative"]) / min(times["learned"])
+    print(f"learned lengths: {shorter} of {len(ps)} shorter; device time {min(times['learned']):.1f} ms against "
+          f"{min(times['conservative']):.1f} ms at the conservative lengths ({gain:.3f}x)", flush=True)
+    assert shorter >= len(ps) // 8
+    assert gain > 1.0, times
