@@ -117,7 +117,7 @@ struct ElemOf {
 // H - P (slot 1; for P = 0, row H/2).
 LL_HD u64 elem_index(u32 h, u32 w, u32 W, u32 H, u32 LC) {
     const u32 hh = H / 2;
-    const u32 P = (h < hh ? h : H - h) & (hh - 1);
+    const u32 P = h < hh ? h : (h == hh ? 0u : H - h);
     return (u64)P * 2 * W + (u64)(w / LC) * (2 * LC) + (h < hh ? 0 : LC) + w % LC;
 }
 

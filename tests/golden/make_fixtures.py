@@ -14,11 +14,13 @@ the low 64 bits of the canonical residue.  Each fixture is plain CSV:
   above_batch_gmp.csv    the four primes just above 21 x 8192 (beyond the batched FP64
                          lengths, so a batch call runs them on the large engine), full test:
                          p,prime,res64
+  concurrent_gmp.csv     the ten exponents of the concurrent large-job test, p,iters,res64 at 400
+                         squarings
   sched_gmp.csv          one exponent per large-engine length N (the largest prime <= 16.5 N),
                          p,iters,res64 at 201 squarings (two graph blocks: the block boundary's
                          carry buffer is crossed once)
 
-Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp|above|rollback|sched [--threads T]
+Usage:  python tests/golden/make_fixtures.py c1|c3|big|c4list|c4gmp|above|rollback|sched|concurrent [--threads T]
 """
 from __future__ import annotations
 
@@ -186,9 +188,22 @@ def sched(path: str, threads: int) -> None:
     print(f"{len(ps)} schedule-test residues in {time.time() - t0:.0f} s")
 
 
+CONCURRENT = [1048573, 1299709, 2000003, 2750159, 4194301, 5000011, 6291469, 8388593, 1572869, 3145739]
+
+
+def concurrent(path: str, threads: int) -> None:
+    out = O.ll_many(CONCURRENT, iters=400, engine=0, threads=threads)
+    with open(path, "w") as f:
+        f.write("p,iters,res64\n")
+        for p, (_, r) in zip(CONCURRENT, out):
+            f.write(f"{p},400,{r:016x}\n")
+    print(f"{len(CONCURRENT)} concurrent-test residues written")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above", "rollback", "sched"])
+    ap.add_argument("what", choices=["c1", "c3", "big", "c4list", "c4gmp", "above", "rollback", "sched",
+                                    "concurrent"])
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--order", choices=["asc", "desc"], default="asc")
@@ -204,6 +219,8 @@ def main() -> None:
         above_batch(os.path.join(HERE, "above_batch_gmp.csv"), a.threads)
     elif a.what == "rollback":
         rollback(os.path.join(HERE, "rollback_gmp.csv"), a.threads)
+    elif a.what == "concurrent":
+        concurrent(os.path.join(HERE, "concurrent_gmp.csv"), a.threads)
     elif a.what == "sched":
         sched(os.path.join(HERE, "sched_gmp.csv"), a.threads)
     elif a.what == "c4list":

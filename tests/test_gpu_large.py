@@ -120,16 +120,19 @@ def test_heldout_exponents_in_one_call_match_gmp():
 
 
 @pytest.mark.parametrize("tr", [llhip.FP64, llhip.NTT], ids=["fp64", "ntt"])
-def test_concurrent_large_jobs_equal_single_runs(tr):
+def test_concurrent_large_jobs_equal_gmp(tr):
     """Ten large exponents at mixed lengths in one call (more jobs than streams, so slots are
-    refilled) give the Res64 of the same exponents run one per call."""
-    ps = [1048573, 1299709, 2000003, 2750159, 4194301, 5000011, 6291469, 8388593, 1572869, 3145739]
-    k = 400
+    refilled) give GMP's Res64 after 400 squarings (tests/golden/concurrent_gmp.csv), and the
+    lengths of the same exponents run one per call."""
+    rows = load_interim_fixture("concurrent_gmp.csv")
+    ps = [p for p, _, _ in rows]
+    k = rows[0][1]
     out = llhip.batch(ps, k, tr)
-    for i, p in enumerate(ps):
+    for i, (p, kk, want) in enumerate(rows):
+        assert kk == k and int(out["status"][i]) == llhip.ST_OK, p
+        assert int(out["res64"][i]) == want, p
         one = llhip.lucas_lehmer_n(p, k, tr, 0)
-        assert one["status"] == llhip.ST_OK and int(out["status"][i]) == llhip.ST_OK, p
-        assert int(out["res64"][i]) == one["res64"], p
+        assert one["status"] == llhip.ST_OK and one["res64"] == want, p
         assert int(out["n_used"][i]) == one["n_used"], p
 
 
