@@ -175,6 +175,17 @@ int llh_lucas_lehmer_n(uint64_t p, uint64_t max_iters, int transform, uint64_t n
  * {1, 3, 5} * 2^k (lucaslehmer.cpp:250-252). */
 size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap);
 
+/* Schedules of the batched engine at length n (digit count): the radix list of each, 4 entries
+ * per schedule (R1..R4 of a power-of-two length; 8, the middle radices, 8 of an odd-factor one;
+ * unused entries 0), index 0 the length's base schedule.  Writes up to cap schedules, returns the
+ * count (0: not a batched length) or a negative status.  llh_set_batch_schedule runs length n at
+ * schedule `sched` from now on (-1: the measured choice compiled into the library);
+ * llh_get_batch_schedule returns the index in force.  No reference counterpart: the reference
+ * fixes its radix-2 schedule and 256-thread blocks (cuda/ll_ntt.cu:99-127, :224). */
+int llh_batch_schedules(int transform, uint64_t n, int32_t *radices, int cap);
+int llh_set_batch_schedule(int transform, uint64_t n, int sched);
+int llh_get_batch_schedule(int transform, uint64_t n);
+
 /* Schedules of the large engine at length n (digit count): the H x W split of its transform
  * elements and the column-pass block width CW.  Writes up to cap entries into w, h, cw (any may be
  * null) and returns the count (0: not a large-engine length), or a negative status.  No reference

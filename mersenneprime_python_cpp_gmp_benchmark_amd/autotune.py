@@ -46,7 +46,7 @@ SCHED_JSON = os.path.join(HERE, "large_schedules.json")
 KERNEL_SOURCES = {
     "large": ["ll_large_impl.hpp", "ll_large_scheds.inc", "ll_large_sched_defaults.inc", "ll_dft.hpp",
               "ll_common.hpp"],
-    "batch": ["ll_batch.hip", "ll_dft.hpp", "ll_common.hpp"],
+    "batch": ["ll_batch.hip", "ll_batch_sched_defaults.inc", "ll_dft.hpp", "ll_common.hpp"],
 }
 
 
@@ -183,6 +183,7 @@ def write_schedules(meas: dict, iters: int) -> dict:
 
 BATCH_JSON = os.path.join(HERE, "batch_costs.json")
 BATCH_INC = os.path.join(HERE, "csrc", "ll_batch_costs.inc")
+BATCH_SCHED_INC = os.path.join(HERE, "csrc", "ll_batch_sched_defaults.inc")
 
 
 def _primes(lo: int, hi: int) -> list:
@@ -207,8 +208,11 @@ def batch_exponent(n: int, transform: int) -> int:
     return ps[-1]
 
 
-def measure_batch(iters: int = 300, transforms=(llhip.FP64, llhip.NTT)) -> dict:
-    out = {}
+def measure_batch(iters: int = 300, transforms=(llhip.FP64, llhip.NTT)) -> tuple[dict, dict]:
+    """(costs, schedules): costs[(tr, n)] = device microseconds per exponent-squaring with the GPU
+    full of exponents of length n at its fastest schedule; schedules[(tr, n)] = [(radices, us)]
+    for every schedule of a length that has several (the choice is the fastest)."""
+    out, scheds = {}, {}
     for tr in transforms:
         lens = [n for n in llhip.supported_lengths(tr) if 32 <= n <= 8192 and llhip.get_batch_cost(n, tr) >= 0]
         for n in lens:
@@ -218,26 +222,51 @@ def measure_batch(iters: int = 300, transforms=(llhip.FP64, llhip.NTT)) -> dict:
             p = batch_exponent(n, tr)
             count = max(2048, min(65536, (8 << 20) // m))
             ps = [p] * count
-            llhip.batch_n(ps[:64], 20, tr, n, no_rollback=True)              # tables + warm-up
-            r = llhip.batch_n(ps, iters, tr, n, no_rollback=True)
-            if not (r["n_used"] == n).all() or not (r["status"] == llhip.ST_OK).all():
+            radices = llhip.batch_schedules(n, tr)
+            rows = []
+            try:
+                for s in range(len(radices)):
+                    llhip.set_batch_schedule(n, s, tr)
+                    llhip.batch_n(ps[:64], 20, tr, n, no_rollback=True)              # tables + warm-up
+                    r = llhip.batch_n(ps, iters, tr, n, no_rollback=True)
+                    if not (r["n_used"] == n).all() or not (r["status"] == llhip.ST_OK).all():
+                        rows = []
+                        break
+                    ms, _ = llhip.last_launch_stats()
+                    rows.append((radices[s], ms * 1e3 / (count * iters)))
+            finally:
+                llhip.set_batch_schedule(n, -1, tr)
+            if not rows:
                 print(f"N={n}: not run at this length", flush=True)
                 continue
-            ms, _ = llhip.last_launch_stats()
-            us = ms * 1e3 / (count * iters)
-            out[(tr, n)] = us
-            print(f"{'fp64' if tr == llhip.FP64 else 'ntt '} N={n:6d} p={p:7d} x{count:6d}: {us:9.5f} us/exponent-squaring",
+            best = min(range(len(rows)), key=lambda i: rows[i][1])
+            out[(tr, n)] = rows[best][1]
+            if len(rows) > 1:
+                scheds[(tr, n)] = (best, rows)
+            print(f"{'fp64' if tr == llhip.FP64 else 'ntt '} N={n:6d} p={p:7d} x{count:6d}: " +
+                  "  ".join(f"{'x'.join(map(str, rad))} {us:9.5f}" for rad, us in rows) + " us/exponent-squaring",
                   flush=True)
-    return out
+    return out, scheds
 
 
-def write_batch(costs: dict, iters: int) -> None:
+def write_batch(costs: dict, scheds: dict, iters: int) -> None:
     dev = llhip.device_info()
+    with open(BATCH_SCHED_INC, "w") as f:
+        f.write("// Measured schedule of each batched length: LLB(transform, N, index into llh_batch_schedules),\n"
+                "// transform 0 FP64, 1 NTT.  Written by `python -m mersenneprime_python_cpp_gmp_benchmark_amd"
+                ".autotune --batch`\n"
+                f"// on {dev['name'] or dev['arch']} ({dev['arch']}): the fastest radix order with the GPU full of "
+                "exponents of that length.\n")
+        for (tr, n), (best, _) in sorted(scheds.items()):
+            f.write(f"LLB({tr}, {n}, {best})\n")
     with open(BATCH_JSON, "w") as f:
         json.dump({"device": dev["name"] or dev["arch"], "arch": dev["arch"], "iters": iters,
                    "kernel_hash": kernel_source_hash("batch"),
                    "us_per_exponent_squaring": {f"{'fp64' if tr == llhip.FP64 else 'ntt'}:{n}": v
-                                                for (tr, n), v in sorted(costs.items())}}, f, indent=1)
+                                                for (tr, n), v in sorted(costs.items())},
+                   "schedules": {f"{'fp64' if tr == llhip.FP64 else 'ntt'}:{n}":
+                                 {"x".join(map(str, rad)): us for rad, us in rows}
+                                 for (tr, n), (_, rows) in sorted(scheds.items())}}, f, indent=1)
     with open(BATCH_INC, "w") as f:
         f.write("// Batched-engine length costs, written by `python -m mersenneprime_python_cpp_gmp_benchmark_amd"
                 ".autotune --batch`\n")
@@ -266,7 +295,8 @@ def main() -> int:
     a = ap.parse_args()
     if a.batch:
         it = a.iters if a.iters != 800 else 300
-        write_batch(measure_batch(it), it)
+        costs, scheds = measure_batch(it)
+        write_batch(costs, scheds, it)
         return 0
     if a.schedules:
         choice = write_schedules(measure_schedules(), 800)

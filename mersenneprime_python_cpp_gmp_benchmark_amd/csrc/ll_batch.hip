@@ -85,8 +85,18 @@ constexpr Cls kCls[] = {
     {896, 112, 8, 448, 0, 0, 0, 0, 7, 2},      // 20
     {1792, 224, 8, 448, 0, 0, 0, 0, 7, 4},     // 21
     {3584, 448, 8, 448, 0, 0, 0, 0, 7, 8},     // 22
+    // Schedule variants: the same length with the middle radices in another order (another
+    // pass span L for each, so other twiddle and LDS patterns).  The host runs a length at the
+    // variant the autotuner measured fastest (kClsBase maps a variant to its length's class).
+    {1024, 128, 8, 256, 8, 8, 2, 8, 0, 0},     // 23: class 7 as 8 8 2 8
+    {2048, 256, 8, 256, 8, 8, 4, 8, 0, 0},     // 24: class 8 as 8 8 4 8
+    {1280, 160, 8, 320, 0, 0, 0, 0, 4, 5},     // 25: class 17 as 8 4 5 8
+    {2560, 320, 8, 320, 0, 0, 0, 0, 8, 5},     // 26: class 18 as 8 8 5 8
 };
 constexpr int kNumCls = (int)(sizeof(kCls) / sizeof(kCls[0]));
+constexpr int kClsBase[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22,
+                            7, 8, 17, 18};
+static_assert(sizeof(kClsBase) / sizeof(kClsBase[0]) == kNumCls, "every class has a base class");
 // Kernel families: 0 the powers of two up to M = 2048 and the one-word lane (one exponent of
 // M = 2048 per 4-wave workgroup, so a barrier waits on that exponent's waves only), 1 3 * 2^k,
 // 2 5 * 2^k, 3 7 * 2^k, 4 the power of two M = 4096 (G = 512).
@@ -704,7 +714,7 @@ ll_batch_kernel(const llh::ClassDesc *__restrict__ classes, int ncls,
             LL_CLASS(1) LL_CLASS(2) LL_CLASS(3) LL_CLASS(4) LL_CLASS(5) LL_CLASS(6) LL_CLASS(7)
             LL_CLASS(8) LL_CLASS(9) LL_CLASS(10) LL_CLASS(11) LL_CLASS(12) LL_CLASS(13) LL_CLASS(14)
             LL_CLASS(15) LL_CLASS(16) LL_CLASS(17) LL_CLASS(18) LL_CLASS(19) LL_CLASS(20) LL_CLASS(21)
-            LL_CLASS(22)
+            LL_CLASS(22) LL_CLASS(23) LL_CLASS(24) LL_CLASS(25) LL_CLASS(26)
 #undef LL_CLASS
             case 0:
                 if constexpr (WG == 256) run_tiny(cd, items, out, counters + c);
@@ -714,7 +724,7 @@ ll_batch_kernel(const llh::ClassDesc *__restrict__ classes, int ncls,
         __syncthreads();
     }
 }
-static_assert(kNumCls == 23, "the class switch lists every class");
+static_assert(kNumCls == 27, "the class switch lists every class");
 
 template <int C>
 static void pass_exponents(std::vector<u32> &ex) {
@@ -729,11 +739,21 @@ namespace llh {
 
 int batch_num_classes() { return kNumCls; }
 
-// class of transform length M (0 = none)
+// class of transform length M (0 = none); the length's base class, not a schedule variant
 int batch_class_of_m(u64 M) {
     for (int c = 1; c < kNumCls; ++c)
-        if ((u64)kCls[c].M == M) return c;
+        if ((u64)kCls[c].M == M && kClsBase[c] == c) return c;
     return 0;
+}
+int batch_class_base(int c) { return c >= 0 && c < kNumCls ? kClsBase[c] : 0; }
+// radix list of class c: R1..R4 (power of two) or 8, H1, [H2], 8; unused entries 0
+void batch_class_radices(int c, int r[4]) {
+    const Cls &k = kCls[c > 0 && c < kNumCls ? c : 0];
+    if (k.H1 == 0) {
+        r[0] = k.R1; r[1] = k.R2; r[2] = k.R3; r[3] = k.R4;
+    } else {
+        r[0] = 8; r[1] = k.H1; r[2] = k.H2 ? k.H2 : 8; r[3] = k.H2 ? 8 : 0;
+    }
 }
 u64 batch_class_m(int c) { return c > 0 && c < kNumCls ? (u64)kCls[c].M : 0; }
 bool batch_class_pow2(int c) { return c > 0 && c < kNumCls && kCls[c].H1 == 0; }
@@ -757,7 +777,7 @@ bool batch_pass_exponents(int c, std::vector<u32> &ex) {
     case C: pass_exponents<C>(ex); return true;
         LL_PX(1) LL_PX(2) LL_PX(3) LL_PX(4) LL_PX(5) LL_PX(6) LL_PX(7) LL_PX(8) LL_PX(9) LL_PX(10) LL_PX(11)
         LL_PX(12) LL_PX(13) LL_PX(14) LL_PX(15) LL_PX(16) LL_PX(17) LL_PX(18) LL_PX(19) LL_PX(20) LL_PX(21)
-        LL_PX(22)
+        LL_PX(22) LL_PX(23) LL_PX(24) LL_PX(25) LL_PX(26)
 #undef LL_PX
         default: return false;
     }

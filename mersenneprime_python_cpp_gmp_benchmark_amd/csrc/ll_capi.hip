@@ -345,24 +345,62 @@ static int get_tables(int transform, int c, TableSet &out) {
     return LLH_OK;
 }
 
-// Batched-engine lengths (N, class), ascending: FP64 N = 2M over every class, GL N = M over
-// the power-of-two classes.
+// Batched-engine lengths (N, base class), ascending: FP64 N = 2M over every length, GL N = M over
+// the power-of-two lengths.  Schedule variants of a length are not listed (batch_class_of picks).
 static const std::vector<std::pair<u64, int>> &batch_ladder(int transform) {
     static const std::vector<std::pair<u64, int>> lad[2] = {
         [] {
             std::vector<std::pair<u64, int>> v;
-            for (int c = 1; c < batch_num_classes(); ++c) v.push_back({2 * batch_class_m(c), c});
+            for (int c = 1; c < batch_num_classes(); ++c)
+                if (batch_class_base(c) == c) v.push_back({2 * batch_class_m(c), c});
             std::sort(v.begin(), v.end());
             return v;
         }(),
         [] {
             std::vector<std::pair<u64, int>> v;
             for (int c = 1; c < batch_num_classes(); ++c)
-                if (batch_class_pow2(c)) v.push_back({batch_class_m(c), c});
+                if (batch_class_pow2(c) && batch_class_base(c) == c) v.push_back({batch_class_m(c), c});
             std::sort(v.begin(), v.end());
             return v;
         }()};
     return lad[transform == kNTT ? 1 : 0];
+}
+
+// The schedules of base class b (b itself first, then its variants in class order); the GL
+// engine runs the power-of-two ones only.
+static std::vector<int> batch_schedules_of(int transform, int b) {
+    std::vector<int> v;
+    if (b <= 0) return v;
+    v.push_back(b);
+    for (int c = 1; c < batch_num_classes(); ++c)
+        if (c != b && batch_class_base(c) == b && (transform == kFP64 || batch_class_pow2(c))) v.push_back(c);
+    return v;
+}
+
+// Schedule of each batched length (index into batch_schedules_of); a length without an entry runs
+// the measured choice compiled in from ll_batch_sched_defaults.inc, else its base class.
+static std::mutex g_bsched_mx;
+static std::map<std::pair<int, u64>, int> g_batch_sched;
+struct BatchSchedDefault {
+    int transform;
+    u64 n;
+    int index;
+};
+static const BatchSchedDefault kBatchSchedDefaults[] = {
+#define LLB(TR, NV, IDX) {TR, NV, IDX},
+#include "ll_batch_sched_defaults.inc"
+#undef LLB
+    {-1, 0, 0}};
+
+static int batch_sched_index(int transform, u64 N) {
+    {
+        std::lock_guard<std::mutex> lk(g_bsched_mx);
+        auto it = g_batch_sched.find(std::make_pair(transform, N));
+        if (it != g_batch_sched.end()) return it->second;
+    }
+    for (const BatchSchedDefault &d : kBatchSchedDefaults)
+        if (d.transform == transform && d.n == N) return d.index;
+    return 0;
 }
 
 // Tables of every batch class of both transforms (the NTT runs the power-of-two classes only).
@@ -379,10 +417,14 @@ static int build_batch_tables() {
 // the kernels index digits with 32-bit products j * p, j < N
 static bool batch_fits(u64 p, u64 N) { return (N + 2) * p < ((u64)1 << 32); }
 
-// Batch class of length N (0 = not a batched length).
+// Batch class that runs length N: its schedule in force (0 = not a batched length).
 static int batch_class_of(int transform, u64 N) {
     for (const auto &e : batch_ladder(transform))
-        if (e.first == N) return e.second;
+        if (e.first == N) {
+            const std::vector<int> v = batch_schedules_of(transform, e.second);
+            const int i = batch_sched_index(transform, N);
+            return i >= 0 && i < (int)v.size() ? v[i] : e.second;
+        }
     return 0;
 }
 
@@ -1579,6 +1621,43 @@ size_t llh_supported_lengths(int transform, uint64_t *out, size_t cap) {
     if (out)
         for (size_t i = 0; i < v.size() && i < cap; ++i) out[i] = v[i];
     return v.size();
+}
+
+int llh_batch_schedules(int transform, uint64_t n, int32_t *radices, int cap) {
+    if (transform != kFP64 && transform != kNTT) {
+        set_error("llh_batch_schedules: unknown transform");
+        return LLH_ERR_ARG;
+    }
+    int b = 0;
+    for (const auto &e : batch_ladder(transform))
+        if (e.first == n) b = e.second;
+    const std::vector<int> v = batch_schedules_of(transform, b);
+    for (int i = 0; i < (int)v.size() && i < cap; ++i)
+        if (radices) batch_class_radices(v[i], radices + 4 * i);
+    return (int)v.size();
+}
+
+int llh_set_batch_schedule(int transform, uint64_t n, int sched) {
+    const int count = llh_batch_schedules(transform, n, nullptr, 0);
+    if (count < 0) return count;
+    if (count == 0 || sched < -1 || sched >= count) {
+        set_error("llh_set_batch_schedule: no such schedule for this length");
+        return LLH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_bsched_mx);
+    if (sched < 0) g_batch_sched.erase(std::make_pair(transform, (u64)n));
+    else g_batch_sched[std::make_pair(transform, (u64)n)] = sched;
+    return LLH_OK;
+}
+
+int llh_get_batch_schedule(int transform, uint64_t n) {
+    const int count = llh_batch_schedules(transform, n, nullptr, 0);
+    if (count <= 0) {
+        if (count == 0) set_error("llh_get_batch_schedule: not a batched length");
+        return count < 0 ? count : LLH_ERR_ARG;
+    }
+    const int i = batch_sched_index(transform, n);
+    return i >= 0 && i < count ? i : 0;
 }
 
 int llh_large_schedules(int transform, uint64_t n, int32_t *w, int32_t *h, int32_t *cw, int cap) {

@@ -87,7 +87,9 @@ constexpr int kWtabSlot = 128;
 
 // ---- batched engine size classes and launches (ll_batch.hip) ------------------------------
 int batch_num_classes();
-int batch_class_of_m(u64 M);            // class of transform length M (0 = none)
+int batch_class_of_m(u64 M);            // class of transform length M (0 = none; the base class)
+int batch_class_base(int c);            // the base class of a schedule variant (itself otherwise)
+void batch_class_radices(int c, int r[4]);   // radix list (unused entries 0)
 u64 batch_class_m(int c);               // transform length of class c
 bool batch_class_pow2(int c);
 int batch_num_families();

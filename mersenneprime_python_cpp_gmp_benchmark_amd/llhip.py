@@ -77,6 +77,9 @@ def lib():
         L.llh_set_checkpoint_blocks.argtypes = [ctypes.c_int]
         L.llh_relayout_host.argtypes = [u64, u64, P(ctypes.c_int64), u64, P(ctypes.c_int64)]
         L.llh_large_schedules.argtypes = [ctypes.c_int, u64, P(i32), P(i32), P(i32), ctypes.c_int]
+        L.llh_batch_schedules.argtypes = [ctypes.c_int, u64, P(i32), ctypes.c_int]
+        L.llh_set_batch_schedule.argtypes = [ctypes.c_int, u64, ctypes.c_int]
+        L.llh_get_batch_schedule.argtypes = [ctypes.c_int, u64]
         L.llh_set_large_schedule.argtypes = [ctypes.c_int, u64, ctypes.c_int]
         L.llh_get_large_schedule.argtypes = [ctypes.c_int, u64]
         _lib = L
@@ -308,6 +311,29 @@ def set_large_schedule(n: int, sched: int, transform: int = FP64) -> None:
 
 def get_large_schedule(n: int, transform: int = FP64) -> int:
     r = lib().llh_get_large_schedule(transform, n)
+    if r < 0:
+        _check(r)
+    return r
+
+
+def batch_schedules(n: int, transform: int = FP64) -> list[tuple[int, ...]]:
+    """Radix list of every batched-engine schedule of length n (index = list position)."""
+    L = lib()
+    cnt = L.llh_batch_schedules(transform, n, None, 0)
+    if cnt < 0:
+        _check(cnt)
+    r = np.zeros(4 * max(cnt, 1), np.int32)
+    L.llh_batch_schedules(transform, n, r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cnt)
+    return [tuple(int(v) for v in r[4 * i:4 * i + 4] if v) for i in range(cnt)]
+
+
+def set_batch_schedule(n: int, sched: int, transform: int = FP64) -> None:
+    """Run batched length n at schedule index `sched` from now on (-1: the library's measured choice)."""
+    _check(lib().llh_set_batch_schedule(transform, n, sched))
+
+
+def get_batch_schedule(n: int, transform: int = FP64) -> int:
+    r = lib().llh_get_batch_schedule(transform, n)
     if r < 0:
         _check(r)
     return r

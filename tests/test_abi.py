@@ -96,3 +96,34 @@ def test_large_schedule_selection_is_host_only_and_checked():
         except RuntimeError:
             continue
         raise AssertionError(f"accepted {bad}")
+
+
+def test_batch_schedule_selection_is_host_only_and_checked():
+    """Batched lengths with several radix orders list them, select them and restore the default."""
+    for n, tr in ((2048, llhip.FP64), (4096, llhip.FP64), (2560, llhip.FP64), (5120, llhip.FP64),
+                  (1024, llhip.NTT), (2048, llhip.NTT)):
+        sch = llhip.batch_schedules(n, tr)
+        m = n // 2 if tr == llhip.FP64 else n
+        assert len(sch) == 2 and len(set(sch)) == 2, (n, sch)
+        for rad in sch:
+            prod = 1
+            for r in rad:
+                prod *= r
+            assert prod == m and rad[0] == 8 and rad[-1] == 8, (n, rad)
+        cur = llhip.get_batch_schedule(n, tr)
+        try:
+            for s in range(len(sch)):
+                llhip.set_batch_schedule(n, s, tr)
+                assert llhip.get_batch_schedule(n, tr) == s
+        finally:
+            llhip.set_batch_schedule(n, -1, tr)
+        assert llhip.get_batch_schedule(n, tr) == cur
+    assert len(llhip.batch_schedules(8192, llhip.FP64)) == 1         # one radix order (all 8s)
+    assert llhip.batch_schedules(5120, llhip.NTT) == []              # no 5 * 2^k NTT
+    assert llhip.batch_schedules(1 << 20, llhip.FP64) == []          # a large-engine length
+    for bad in ((2048, 2, llhip.FP64), (1 << 20, 0, llhip.FP64), (2048, -2, llhip.NTT)):
+        try:
+            llhip.set_batch_schedule(*bad)
+        except RuntimeError:
+            continue
+        raise AssertionError(f"accepted {bad}")

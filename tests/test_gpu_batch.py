@@ -191,3 +191,38 @@ def test_rollbacks_resume_from_checkpoints():
     assert st["rollbacks"] > 0 and st["resumed"] == st["rollbacks"], st
     assert total <= st["squarings"] <= total + 64 * st["rollbacks"], (st, total)
     assert st0["rollbacks"] > 0 and st0["resumed"] == 0 and st0["squarings"] > st["squarings"]
+
+
+def test_every_batch_schedule_matches_gmp():
+    """Every schedule (radix order) of every batched length that has more than one, on both
+    transforms where it runs: full tests of twelve primes at that length (p / N from 16 bits to
+    the cap, or to the NTT's exactness bound) give GMP's verdicts and Res64 (the C3 fixture).
+    The schedule in force is restored after each length."""
+    fx = {p: (v, r) for p, v, r in load_range_fixture("c3_gmp_p100001.csv")}
+    primes = sorted(fx)
+    L = llhip.lib()
+    checked = 0
+    for tr in (llhip.FP64, llhip.NTT):
+        for n in llhip.supported_lengths(tr):
+            scheds = llhip.batch_schedules(n, tr)
+            if len(scheds) < 2:
+                continue
+            if tr == llhip.FP64:
+                lo, hi = 16 * n, int(llhip.get_batch_bpw() * n)
+            else:
+                bits = (63 - (n.bit_length() - 1)) // 2
+                lo, hi = int(0.7 * bits * n), bits * n - n // 2
+            cand = [p for p in primes if lo <= p <= min(hi, 100001)]
+            ps = cand[:: max(1, len(cand) // 12)][:12]
+            assert len(ps) >= 6, (tr, n)
+            try:
+                for s, rad in enumerate(scheds):
+                    llhip.set_batch_schedule(n, s, tr)
+                    out = llhip.batch_n(ps, 0, tr, n, no_rollback=True)
+                    for i, p in enumerate(ps):
+                        assert int(out["status"][i]) == llhip.ST_OK and int(out["n_used"][i]) == n, (tr, n, rad, p)
+                        assert (int(out["verdict"][i] != 0), int(out["res64"][i])) == fx[p], (tr, n, rad, p)
+                    checked += 1
+            finally:
+                llhip.set_batch_schedule(n, -1, tr)
+    assert checked >= 10
