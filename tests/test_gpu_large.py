@@ -74,6 +74,18 @@ def test_interim_fixtures():
                 assert r["n_used"] == 1 << 20
 
 
+@pytest.mark.parametrize("n", [7 << 17, 15 << 16, 1 << 20], ids=["N917504", "N983040", "N1048576"])
+def test_p16777213_5000_at_learned_lengths(n):
+    """p = 16,777,213 @ 5,000 squarings at the shorter FP64 lengths the round-off model may choose
+    (7 * 2^17 at 18.3 bits per digit, 15 * 2^16 at 17.1) and at the conservative 2^20: every Res64
+    equals the GMP fixture, with no rollback and round-off under the limit."""
+    want = {p: r for p, k, r in load_interim_fixture("big_gmp_interim.csv") if k == 5000}[16777213]
+    r = llhip.lucas_lehmer_n(16777213, 5000, llhip.FP64, n, no_rollback=True)
+    assert r["status"] == llhip.ST_OK and r["n_used"] == n, r
+    assert r["res64"] == want
+    assert r["maxerr"] < 0.4
+
+
 def test_default_length_routes_large_exponents():
     # exponents beyond the batched engine go to the large engine inside one llh_batch call
     ps = [99991, 1048573, 2000003]
