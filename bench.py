@@ -22,7 +22,7 @@ exponent list; residues are created and stay on the GPU), plus
                 p = 1,048,573 (N = 2^16), p = 16,777,213 on the FP64 engine at the conservative
                 length and at the learned length, the same on the exact NTT, the round-off
                 model's 64 held-out exponents (plain one-per-call conservative form against one
-                call at the learned lengths, 5000 squarings each), and the plain
+                call at the learned lengths, 20,000 squarings each, Res64 against GMP), and the plain
                 one-exponent-per-launch form of the batch.
 
 --workload selects one configuration alone (used for the rocprofv3 runs).  The library
@@ -405,11 +405,12 @@ def main() -> int:
                                                           if (P24, k24) in interim else None)
         extra["p16777213_5000_iters"] = ent
         # C4: the round-off model's held-out exponents (primes drawn with seed 0 from
-        # [2^20, 2^25)): plain GPU form (one per call, conservative length) against one call at
-        # the learned lengths
+        # [2^20, 2^25)) at their configured 20,000 squarings: plain GPU form (one per call,
+        # conservative length) against one call at the conservative and at the learned lengths,
+        # every Res64 checked against the GMP fixture
         try:
             from mersenneprime_python_cpp_gmp_benchmark_amd import c4_eval
-            extra["c4_heldout_5000_iters"] = c4_eval.compare(5000, 0, True, log=log)["summary"]
+            extra["c4_heldout_20000_iters"] = c4_eval.compare(20000, 0, True, log=log)["summary"]
         except Exception as e:  # noqa: BLE001 -- the model or its data may be absent
             log(f"held-out comparison skipped: {e}")
         # plain GPU form of the batch: one exponent per launch, a sample across the range
