@@ -693,6 +693,16 @@ static int large_sched_of(int transform, u64 N) {
     return 0;
 }
 
+// A transform's twiddles w^j (j < n, `words` values each) rearranged into the per-pass order the
+// kernels read: entry e holds w^{exp(e)}, e < size.
+template <class V>
+static std::vector<V> pass_order(const std::vector<V> &nat, int words, u32 (*exp)(int), int size) {
+    std::vector<V> v((size_t)size * words);
+    for (int e = 0; e < size; ++e)
+        for (int k = 0; k < words; ++k) v[(size_t)e * words + k] = nat[(size_t)exp(e) * words + k];
+    return v;
+}
+
 static int get_large_tables(int transform, u64 N, int sched, LargeTables &out) {
     const u64 M0 = transform == kFP64 ? N / 2 : N;
     const std::vector<LargeSched> sv = large_schedules(transform, M0);
@@ -730,8 +740,8 @@ static int get_large_tables(int transform, u64 N, int sched, LargeTables &out) {
             }
             return v;
         };
-        if ((rc = upload(&T.twW, roots(W, W, 1)))) return rc;
-        if ((rc = upload(&T.twH, roots(H, H, 1)))) return rc;
+        if ((rc = upload(&T.twW, pass_order(roots(W, W, 1), 2, L.row_exp, L.row_tab)))) return rc;
+        if ((rc = upload(&T.twH, pass_order(roots(H, H, 1), 2, L.col_exp, L.col_tab)))) return rc;
         if ((rc = upload(&T.tmL, roots(M, S, 1)))) return rc;
         if ((rc = upload(&T.tmH, roots(M, nS, S)))) return rc;
         std::vector<double> wfL(Sw), wfH(nSw), wiL(Sw), wiH(nSw);
@@ -762,8 +772,10 @@ static int get_large_tables(int transform, u64 N, int sched, LargeTables &out) {
             return v;
         };
         const u64 wW = gl_pow_h(w, H), wH = gl_pow_h(w, W);   // w_W = w_M^H, w_H = w_M^W
-        if ((rc = upload(&T.twW, powers(wW, W, 1))) || (rc = upload(&T.twWi, powers(gl_inv_h(wW), W, 1))) ||
-            (rc = upload(&T.twH, powers(wH, H, 1))) || (rc = upload(&T.twHi, powers(gl_inv_h(wH), H, 1))) ||
+        if ((rc = upload(&T.twW, pass_order(powers(wW, W, 1), 1, L.row_exp, L.row_tab))) ||
+            (rc = upload(&T.twWi, pass_order(powers(gl_inv_h(wW), W, 1), 1, L.row_exp, L.row_tab))) ||
+            (rc = upload(&T.twH, pass_order(powers(wH, H, 1), 1, L.col_exp, L.col_tab))) ||
+            (rc = upload(&T.twHi, pass_order(powers(gl_inv_h(wH), H, 1), 1, L.col_exp, L.col_tab))) ||
             (rc = upload(&T.tmL, powers(w, S, 1))) || (rc = upload(&T.tmH, powers(gl_pow_h(w, S), nS, 1))) ||
             (rc = upload(&T.tmLi, powers(wv, S, 1))) || (rc = upload(&T.tmHi, powers(gl_pow_h(wv, S), nS, 1))) ||
             (rc = upload(&T.wfL, powers(r2, Sw, 1))) || (rc = upload(&T.wfH, powers(gl_pow_h(r2, Sw), nSw, 1))) ||

@@ -107,8 +107,8 @@ struct LargeArgs {
     void *elem;              // [M] elements
     void *carry;             // [nblk][H] row-segment carry-outs (double or i64)
     unsigned long long *err; // per-iteration max round-off slots (double bits)
-    const void *twW, *twWi;  // w_W^j, j < W   (GL: forward, inverse)
-    const void *twH, *twHi;  // w_H^j, j < H
+    const void *twW, *twWi;  // twiddles of the W-point row transform in per-pass order (GL: forward, inverse)
+    const void *twH, *twHi;  // the same for the H-point column transform
     const void *tmL, *tmH;   // w_M^e = tmL[e & (S-1)] * tmH[e >> s]
     const void *tmLi, *tmHi; // GL inverse of the above
     const void *wfL, *wfH;   // forward weight of exponent e: wfL[e & (Sw-1)] * wfH[e >> sw]
@@ -133,6 +133,11 @@ struct LargeLaunch {
     void (*apply)(LargeArgs);
     int w_len, h_len;   // W and H (W carries the odd factor)
     int cw;             // columns per column-pass workgroup
+    // Per-pass twiddle tables of the row and column transforms: entry e (e < *_tab) holds
+    // w_W^{row_exp(e)} (w_H^{col_exp(e)}), the order in which the kernels' passes read them.
+    u32 (*row_exp)(int e);
+    u32 (*col_exp)(int e);
+    int row_tab, col_tab;
 };
 
 // One schedule of the large engine: the H x W split of the M elements and the column-pass

@@ -28,7 +28,10 @@
 // Digits: FP64 packs digits 2j, 2j+1 into element j (N = 2M); GL holds digit j in element j
 // (N = M).  Weights and the w_M twiddles come from two-level tables (product of a low and
 // a high table entry), so every table stays cache resident at every length; the FFT
-// twiddles are staged in LDS in per-pass order.
+// twiddles come from global tables in per-pass order (a pass's lanes read consecutive
+// entries), loaded before each pass's barrier.  Staging them in LDS instead cost every
+// workgroup its own copy and 8-16 KB of LDS, and one squaring took 30.3 us at N = 2^20 against
+// 27.8 us read from L2.
 #include "ll_common.hpp"
 #include "ll_dft.hpp"
 #include "ll_internal.hpp"
@@ -273,17 +276,27 @@ LL_DEV u64 wait_tagged(const u64 *p, u32 seq, unsigned *fault) {
     }
 }
 
-// Per-pass twiddle tables of the H-point and W-point transforms, copied into LDS out of the full
-// tables (all NT threads of the workgroup).
-template <class T, int H, int NT>
-LL_DEV void stage_col_table(T *dst, const void *src, int tid) {
+// Exponents of the per-pass twiddle tables of the H-point and W-point transforms (host side:
+// the host builds the tables in this order, so kernel and table cannot disagree).
+template <int H>
+u32 col_pass_exponent(int e) {
     typedef ColRadix<H> RR;
-    stage_pass_table<T, NT, RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>(dst, src, tid);
+    return PassTable<RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>::exponent(e);
 }
-template <class T, int W, int NT>
-LL_DEV void stage_row_table(T *dst, const void *src, int tid) {
+template <int W>
+u32 row_pass_exponent(int e) {
     typedef RowRadix<W> RR;
-    stage_pass_table<T, NT, RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>(dst, src, tid);
+    return PassTable<RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>::exponent(e);
+}
+template <int H>
+constexpr int col_pass_size() {
+    typedef ColRadix<H> RR;
+    return PassTable<RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>::SIZE;
+}
+template <int W>
+constexpr int row_pass_size() {
+    typedef RowRadix<W> RR;
+    return PassTable<RR::R1, RR::R2, RR::R3, RR::R4, RR::R5>::SIZE;
 }
 
 // ---- column kernels ---------------------------------------------------------------------------
@@ -520,7 +533,6 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     typedef typename C::CT CT;
     typedef typename C::WT WT;
     __shared__ __attribute__((aligned(16))) T lds_all[CW * C::REG];
-    __shared__ __attribute__((aligned(16))) T s_tw[H];
     const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
     const int blk = blockIdx.x, w = blk * CW + c;
     const T *elem = reinterpret_cast<const T *>(A.elem);
@@ -540,11 +552,10 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     col_weights<IS_GL, W, H, CW>(wf, gtab<WT>(A.wfL, A.wfH, A.log_sw), A, w, t);
     T tm[kR];
     col_twiddles<IS_GL, W, H, CW>(tm, gtab<T>(A.tmL, A.tmH, A.log_s), w, t);
-    stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
 #pragma unroll
     for (int s = 0; s < kR; ++s) lds_all[(t + C::GH * s) * CW + c] = x[s];
     __syncthreads();
-    col_forward<IS_GL, W, H, CW>(lds_all, (const T *)s_tw, wf, tm, A, blk, c, t, tid, cin);
+    col_forward<IS_GL, W, H, CW>(lds_all, as_global<T>(A.twH), wf, tm, A, blk, c, t, tid, cin);
 }
 
 // ---- K3: inverse column pass (last squaring of a block: carry-outs to the carry buffer) ---------
@@ -555,7 +566,6 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_inv(LargeArgs A) {
     typedef typename C::CT CT;
     typedef typename C::WT WT;
     __shared__ __attribute__((aligned(16))) T lds_all[CW * C::REG];
-    __shared__ __attribute__((aligned(16))) T s_tw[H];
     __shared__ unsigned long long s_err;
     const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
     const int blk = blockIdx.x, w = blk * CW + c;
@@ -566,10 +576,9 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_inv(LargeArgs A) {
     col_load_inv<IS_GL, W, H, CW>(x, tmi, A, w, t);
     WT wi[C::NWT];
     col_weights<IS_GL, W, H, CW>(wi, gtab<WT>(A.wiL, A.wiH, A.log_sw), A, w, t);
-    stage_col_table<T, H, C::NT>(s_tw, IS_GL ? A.twHi : A.twH, tid);
     CT cy[C::NROW];
     double eloc = 0.0;
-    col_inverse<IS_GL, W, H, CW>(x, lds_all, (const T *)s_tw, wi, A, blk, c, t, tid, cy, eloc);
+    col_inverse<IS_GL, W, H, CW>(x, lds_all, as_global<T>(IS_GL ? A.twHi : A.twH), wi, A, blk, c, t, tid, cy, eloc);
     T *elem = reinterpret_cast<T *>(A.elem);
     CT *carry = reinterpret_cast<CT *>(A.carry);
 #pragma unroll
@@ -616,15 +625,12 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fused(LargeArgs A) {
     typedef typename C::WT WT;
     constexpr int NBLK = C::NBLK;
     __shared__ __attribute__((aligned(16))) T lds_all[CW * C::REG];
-    __shared__ __attribute__((aligned(16))) T s_tw[H];
-    __shared__ __attribute__((aligned(16))) T s_twi[IS_GL ? H : 1];
     __shared__ unsigned long long s_err;
     const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
     const int blk = A.xcd_blocks ? xcd_block<NBLK>(blockIdx.x) : (int)blockIdx.x, w = blk * CW + c;
     if (tid == 0) s_err = 0;
     // Every global load of the thread goes out before the first wait: the element, twiddle and
-    // weight loads, then the twiddle table staged into LDS, whose stores wait for all of them
-    // (staged first, the table's round trip came before the others were issued).
+    // weight loads; each FFT pass then loads its table twiddles before its barrier.
     T tm[kR];
     col_twiddles<IS_GL, W, H, CW>(tm, gtab<T>(A.tmL, A.tmH, A.log_s), w, t);
     T x[kR];
@@ -638,11 +644,9 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fused(LargeArgs A) {
     WT wi[C::NWT], wf[C::NWT];
     col_weights<IS_GL, W, H, CW>(wi, gtab<WT>(A.wiL, A.wiH, A.log_sw), A, w, t);
     col_weights<IS_GL, W, H, CW>(wf, gtab<WT>(A.wfL, A.wfH, A.log_sw), A, w, t);
-    stage_col_table<T, H, C::NT>(s_tw, A.twH, tid);
-    if constexpr (IS_GL) stage_col_table<T, H, C::NT>(s_twi, A.twHi, tid);
     CT cy[C::NROW];
     double eloc = 0.0;
-    col_inverse<IS_GL, W, H, CW>(x, lds_all, (const T *)(IS_GL ? s_twi : s_tw), wi, A, blk, c, t, tid, cy, eloc);
+    col_inverse<IS_GL, W, H, CW>(x, lds_all, as_global<T>(IS_GL ? A.twHi : A.twH), wi, A, blk, c, t, tid, cy, eloc);
     const u32 seq = A.tag;
     u64 *tc = reinterpret_cast<u64 *>(A.tcarry);
     bool wide = false;
@@ -670,7 +674,7 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fused(LargeArgs A) {
         if (!tag_ok(cwd[i], seq)) cwd[i] = wait_tagged(tc + src[i], seq, A.fault);
         carry_from(untag(cwd[i]), cin[i]);
     }
-    col_forward<IS_GL, W, H, CW>(lds_all, (const T *)s_tw, wf, tm, A, blk, c, t, tid, cin);
+    col_forward<IS_GL, W, H, CW>(lds_all, as_global<T>(A.twH), wf, tm, A, blk, c, t, tid, cin);
     // the workgroup's round-off maximum (s_err, complete after the barriers of col_forward) goes
     // out last, off the path of the carry hand-off, with no barrier of its own
     if constexpr (!IS_GL)
@@ -697,8 +701,6 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
     constexpr int GW = W / kR, NT = 2 * GW;
     constexpr int REG = W + W / 8 + 4;
     __shared__ __attribute__((aligned(16))) T lds_all[2 * REG];
-    __shared__ __attribute__((aligned(16))) T s_tw[W];
-    __shared__ __attribute__((aligned(16))) T s_twi[IS_GL ? W : 1];
     const int tid = threadIdx.x, g = tid / GW, t = tid % GW;
     const int i = blockIdx.x;
     const int row = g == 0 ? i : (i == 0 ? H / 2 : H - i);
@@ -716,13 +718,11 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
 #pragma unroll
         for (int q = 0; q < kR; ++q) wk[q] = tw2<C64>(A.tmL, A.tmH, (u32)row + (u32)H * (u32)(t + GW * q), A.log_s);
     }
-    stage_row_table<T, W, NT>(s_tw, A.twW, tid);
-    if constexpr (IS_GL) stage_row_table<T, W, NT>(s_twi, A.twWi, tid);
-    row_transform<F, W, GW, false>(x, lds, t, (const T *)s_tw);
+    row_transform<F, W, GW, false>(x, lds, t, as_global<T>(A.twW));
     if constexpr (IS_GL) {
 #pragma unroll
         for (int q = 0; q < kR; ++q) x[q] = gl_sqr(x[q]);
-        row_transform<F, W, GW, true>(x, lds, t, (const T *)s_twi);
+        row_transform<F, W, GW, true>(x, lds, t, as_global<T>(A.twWi));
     } else {
         const int tt = ll_opaque(t);
         __syncthreads();
@@ -743,7 +743,7 @@ __global__ void __launch_bounds__(2 * (W / kR)) lg_rows(LargeArgs A) {
             const C64 Z2 = csqr(x[q]);
             x[q] = cmk(__fma_rn(-0.25, AD.re, Z2.re), __fma_rn(-0.25, AD.im, Z2.im));
         }
-        row_transform<F, W, GW, true>(x, lds, t, (const T *)s_tw);
+        row_transform<F, W, GW, true>(x, lds, t, as_global<T>(A.twW));
     }
 #pragma unroll
     for (int s = 0; s < kR; ++s) st_stream(elem + tl + 2 * GW * s, x[s]);
@@ -792,10 +792,12 @@ __global__ void __launch_bounds__(256) lg_apply_carry(LargeArgs A) {
 }  // namespace
 
 namespace {
-// The five kernels of one schedule (the row pass depends on the split only).
+// The five kernels of one schedule (the row pass depends on the split only) and the order of its
+// twiddle tables.
 template <bool IS_GL, int W, int H, int CW>
 llh::LargeLaunch make_large_launch() {
     return llh::LargeLaunch{lg_cols_fwd<IS_GL, W, H, CW>, lg_rows<IS_GL, W, H>, lg_cols_inv<IS_GL, W, H, CW>,
-                            lg_cols_fused<IS_GL, W, H, CW>, lg_apply_carry<IS_GL, W, H, CW>, W, H, CW};
+                            lg_cols_fused<IS_GL, W, H, CW>, lg_apply_carry<IS_GL, W, H, CW>, W, H, CW,
+                            row_pass_exponent<W>, col_pass_exponent<H>, row_pass_size<W>(), col_pass_size<H>()};
 }
 }  // namespace
