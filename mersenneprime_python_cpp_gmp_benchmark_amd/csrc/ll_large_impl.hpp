@@ -519,6 +519,18 @@ LL_DEV void col_load_inv(typename Col<IS_GL, W, H, CW>::T (&x)[kR], const typena
     }
 }
 
+// Lanes per segment of the round-off reduction of an NT-thread workgroup: the largest power of
+// two up to 64 that divides NT, so every segment lies wholly inside the workgroup and no lane
+// reads a lane that is not there.
+LL_HD constexpr int seg_lanes(int nt) { return nt % 64 == 0 ? 64 : nt % 32 == 0 ? 32 : nt % 16 == 0 ? 16 : nt % 8 == 0 ? 8 : 1; }
+// maximum of v over each segment of SEG lanes (every lane gets its segment's)
+template <int SEG>
+LL_DEV double seg_max(double v) {
+#pragma unroll
+    for (int off = SEG / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, SEG));
+    return v;
+}
+
 LL_DEV void err_publish(unsigned long long *s_err, double eloc, const LargeArgs &A, int tid) {
     atomicMax(s_err, (unsigned long long)__double_as_longlong(eloc));
     __syncthreads();
@@ -658,7 +670,14 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fused(LargeArgs A) {
         st_coh(tc + blk * H + tid + C::NT * i, tag_word(seq, ci));
     }
     if (wide) atomicOr(A.fault, 2u);            // a carry the tagged word cannot hold: fail the run
-    if constexpr (!IS_GL) atomicMax(&s_err, (unsigned long long)__double_as_longlong(eloc));
+    // the workgroup's round-off maximum: reduced over segments of up to a wave first, then one
+    // LDS atomic per segment (one per thread queued the whole workgroup's atomics on one LDS
+    // word, in front of the forward pass's LDS traffic)
+    if constexpr (!IS_GL) {
+        constexpr int SEG = seg_lanes(C::NT);
+        const double em = seg_max<SEG>(eloc);
+        if (tid % SEG == 0) atomicMax(&s_err, (unsigned long long)__double_as_longlong(em));
+    }
     // the first poll of every word is issued before any is waited on: one round trip, not NROW
     CT cin[C::NROW];
     u64 cwd[C::NROW];
