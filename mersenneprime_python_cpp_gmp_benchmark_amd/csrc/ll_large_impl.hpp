@@ -319,6 +319,11 @@ struct Col {
     typedef ColSched<H> S;
     static constexpr int GH = H / kR, NT = CW * GH, NBLK = W / CW;
     static constexpr int REG = H + H / 8 + (IS_GL ? 8 : 4);
+    // Row-segment layout in LDS: segment h at h RSS, RSS = CW + 1 elements (odd), so the lanes of
+    // a wave, which walk consecutive segments, start on distinct banks (at a stride of CW the
+    // 16-byte reads of 16 lanes fell on the same four banks)
+    static constexpr int RSS = CW + 1;
+    static constexpr int LDS = CW * REG > H * RSS ? CW * REG : H * RSS;   // elements of lds_all
     static constexpr int NROW = nrow(CW);
     static constexpr int NWT = nwt(IS_GL);
     // every thread walks NROW segments (CW <= kR), or the first H threads one each (CW > kR)
@@ -396,7 +401,7 @@ LL_DEV void col_inverse(typename Col<IS_GL, W, H, CW>::T (&x)[kR], typename Col<
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kR; ++q) lds_all[(t + C::GH * q) * CW + c] = x[q];
+    for (int q = 0; q < kR; ++q) lds_all[(t + C::GH * q) * C::RSS + c] = x[q];
     __syncthreads();
     const ExpWalk EW = ewalk_of(A);
     const NDiv D = ndiv_of(A);
@@ -405,7 +410,7 @@ LL_DEV void col_inverse(typename Col<IS_GL, W, H, CW>::T (&x)[kR], typename Col<
         const int h = tid + C::NT * i;
         cy[i] = 0;
         if (!C::owns(h)) continue;
-        T *row = lds_all + h * CW;
+        T *row = lds_all + h * C::RSS;
         const u64 j0 = (u64)blk * CW + (u64)W * h;
         u32 e = EW.of_residue(rmod(IS_GL ? j0 : 2 * j0, A.p, D));
         CT cv = 0;
@@ -452,7 +457,7 @@ LL_DEV void col_forward(typename Col<IS_GL, W, H, CW>::T *lds_all, TP tw,
     for (int i = 0; i < C::NROW; ++i) {
         const int h = tid + C::NT * i;
         if (!C::owns(h)) continue;
-        T *row = lds_all + h * CW;
+        T *row = lds_all + h * C::RSS;
         const u64 j0 = (u64)blk * CW + (u64)W * h;
         u32 e = EW.of_residue(rmod(IS_GL ? j0 : 2 * j0, A.p, D));
         CT cv = cin[i];
@@ -486,7 +491,7 @@ LL_DEV void col_forward(typename Col<IS_GL, W, H, CW>::T *lds_all, TP tw,
     T x[kR];
 #pragma unroll
     for (int s = 0; s < kR; ++s) {
-        const T v = lds_all[(t + C::GH * s) * CW + c];
+        const T v = lds_all[(t + C::GH * s) * C::RSS + c];
         if constexpr (IS_GL) x[s] = gl_mul(gl_from_i64((i64)v), wf[s]);
         else x[s] = cmk(v.re * wf[2 * s], v.im * wf[2 * s + 1]);
     }
@@ -531,8 +536,11 @@ LL_DEV double seg_max(double v) {
     return v;
 }
 
+template <int NT>
 LL_DEV void err_publish(unsigned long long *s_err, double eloc, const LargeArgs &A, int tid) {
-    atomicMax(s_err, (unsigned long long)__double_as_longlong(eloc));
+    constexpr int SEG = seg_lanes(NT);
+    const double em = seg_max<SEG>(eloc);
+    if (tid % SEG == 0) atomicMax(s_err, (unsigned long long)__double_as_longlong(em));
     __syncthreads();
     if (tid == 0) atomicMax(A.err + A.err_slot, *s_err);
 }
@@ -544,7 +552,7 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     typedef typename C::T T;
     typedef typename C::CT CT;
     typedef typename C::WT WT;
-    __shared__ __attribute__((aligned(16))) T lds_all[CW * C::REG];
+    __shared__ __attribute__((aligned(16))) T lds_all[C::LDS];
     const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
     const int blk = blockIdx.x, w = blk * CW + c;
     const T *elem = reinterpret_cast<const T *>(A.elem);
@@ -565,7 +573,7 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fwd(LargeArgs A) {
     T tm[kR];
     col_twiddles<IS_GL, W, H, CW>(tm, gtab<T>(A.tmL, A.tmH, A.log_s), w, t);
 #pragma unroll
-    for (int s = 0; s < kR; ++s) lds_all[(t + C::GH * s) * CW + c] = x[s];
+    for (int s = 0; s < kR; ++s) lds_all[(t + C::GH * s) * C::RSS + c] = x[s];
     __syncthreads();
     col_forward<IS_GL, W, H, CW>(lds_all, as_global<T>(A.twH), wf, tm, A, blk, c, t, tid, cin);
 }
@@ -577,7 +585,7 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_inv(LargeArgs A) {
     typedef typename C::T T;
     typedef typename C::CT CT;
     typedef typename C::WT WT;
-    __shared__ __attribute__((aligned(16))) T lds_all[CW * C::REG];
+    __shared__ __attribute__((aligned(16))) T lds_all[C::LDS];
     __shared__ unsigned long long s_err;
     const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
     const int blk = blockIdx.x, w = blk * CW + c;
@@ -598,11 +606,11 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_inv(LargeArgs A) {
         const int h = tid + C::NT * i;
         if (!C::owns(h)) continue;
         carry[blk * H + h] = cy[i];
-        const T *row = lds_all + h * CW;
+        const T *row = lds_all + h * C::RSS;
 #pragma unroll
         for (int cc = 0; cc < CW; ++cc) elem[elem_index(h, blk * CW + cc, W, H, C::E::LC)] = row[cc];
     }
-    if constexpr (!IS_GL) err_publish(&s_err, eloc, A, tid);
+    if constexpr (!IS_GL) err_publish<C::NT>(&s_err, eloc, A, tid);
 }
 
 // ---- K3+K1 fused: inverse column pass of one squaring and forward column pass of the next --------
@@ -636,7 +644,7 @@ __global__ void __launch_bounds__(CW * (H / kR)) lg_cols_fused(LargeArgs A) {
     typedef typename C::CT CT;
     typedef typename C::WT WT;
     constexpr int NBLK = C::NBLK;
-    __shared__ __attribute__((aligned(16))) T lds_all[CW * C::REG];
+    __shared__ __attribute__((aligned(16))) T lds_all[C::LDS];
     __shared__ unsigned long long s_err;
     const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
     const int blk = A.xcd_blocks ? xcd_block<NBLK>(blockIdx.x) : (int)blockIdx.x, w = blk * CW + c;
