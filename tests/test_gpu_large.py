@@ -54,6 +54,17 @@ def test_known_prime_full_test_756839():
     assert r["status"] == llhip.ST_OK and r["verdict"] == 1 and r["res64"] == 0, r
 
 
+def test_known_primes_to_1398269_full_tests_in_one_call():
+    """M859433, M1257787 and M1398269 (known_mp_list.txt) in one call on the large engine at the
+    default FP64 lengths, several in flight: every verdict prime and Res64 0 (about 15 s)."""
+    ps = [859433, 1257787, 1398269]
+    out = llhip.batch(ps, 0, llhip.FP64)
+    assert list(out["status"]) == [llhip.ST_OK] * 3
+    assert [int(v) for v in out["verdict"]] == [1, 1, 1]
+    assert [int(r) for r in out["res64"]] == [0, 0, 0]
+    assert float(max(out["maxerr"])) < 0.4
+
+
 def test_composite_full_test_matches_gmp():
     p = 216071          # prime exponent, M_p composite
     want = O.ll_gmp(p)
